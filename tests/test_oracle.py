@@ -1,0 +1,329 @@
+"""Pins of the CPU oracle (oracle/jw_oracle.cpp), CPU only.
+
+Each check is chosen so that a plausible mistake in the oracle fails it:
+  * against the reference's own per-site code built from its sources
+    (oracle/_ref, N = 64): bit-level agreement;
+  * against numpy.fft / brute-force DFT for the transform conventions;
+  * against mpmath's 50-digit matrix exponential and closed forms for exp(iH);
+  * against the textbook Gell-Mann matrices for the noise colour basis;
+  * against published Philox4x32-10 known-answer vectors;
+  * the full Langevin step / MV init / measurement against the reference's
+    per-site functions composed as main_optimized.cpp:178-191, 243-333 does,
+    with numpy's FFT in the reference's unitary convention.
+"""
+import numpy as np
+import pytest
+
+import oracle as O_mod
+
+N64 = 64
+rng = np.random.default_rng(20261202)
+
+# Textbook Gell-Mann matrices (standard basis, independent of the code under test)
+L = np.zeros((8, 3, 3), complex)
+L[0][0, 1] = L[0][1, 0] = 1
+L[1][0, 1], L[1][1, 0] = -1j, 1j
+L[2][0, 0], L[2][1, 1] = 1, -1
+L[3][0, 2] = L[3][2, 0] = 1
+L[4][0, 2], L[4][2, 0] = -1j, 1j
+L[5][1, 2] = L[5][2, 1] = 1
+L[6][1, 2], L[6][2, 1] = -1j, 1j
+L[7] = np.diag([1, 1, -2]) / np.sqrt(3)
+# the reference's basis flips the antisymmetric ones (field.h:2183-2186, 2204-2207, 2218-2221)
+LT = L * np.array([1, -1, 1, 1, -1, 1, -1, 1])[:, None, None]
+
+
+def herm(scale=1.0, traceless=True):
+    a = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    h = (a + a.conj().T) / 2
+    if traceless:
+        h -= np.trace(h) / 3 * np.eye(3)
+    return h * scale
+
+
+def random_su3_field(orc, N):
+    return np.array([orc.su3_exponentiate(herm(rng.uniform(0.1, 3)), 1.0).reshape(9) for _ in range(N * N)])
+
+
+def npfft(f, N, d):
+    """Reference convention (mpi_fftw_class.h:191-315) via numpy: dir 1 = e^{-i}, both /sqrt(V)."""
+    a = f.reshape(N, N, -1)
+    out = np.fft.fft2(a, axes=(0, 1)) / N if d == 1 else np.fft.ifft2(a, axes=(0, 1)) * N
+    return out.reshape(N * N, -1)
+
+
+# ---------------------------------------------------------------- exponential
+def test_exp_matches_reference_build(orc, ref):
+    worst = 0.0
+    for _ in range(3000):
+        h = herm(rng.uniform(1e-4, 8.0), traceless=bool(rng.integers(2)))
+        # non-Hermitian input: only the upper triangle and real diagonal count
+        h = h + (rng.normal(size=(3, 3)) * 1e-3 * np.tril(np.ones((3, 3)), -1))
+        for s in (1.0, -1.0):
+            worst = max(worst, np.abs(orc.su3_exponentiate(h, s) - ref.su3_exponentiate(h, s)).max())
+    assert worst <= 4e-15
+
+
+@pytest.mark.parametrize("h", [
+    np.zeros((3, 3)), np.diag([1.0, 1.0, -2.0]), np.diag([0.3, -0.1, -0.2]),
+    np.array([[0, 1, 0], [1, 0, 0], [0, 0, 0]], complex), 0.7 * L[7], 1e-9 * L[0] + 2 * L[2],
+])
+def test_exp_degenerate_fallback_matches_reference(orc, ref, h):
+    for s in (1.0, -1.0):
+        assert np.abs(orc.su3_exponentiate(h, s) - ref.su3_exponentiate(h, s)).max() <= 1e-15
+
+
+def test_exp_against_mpmath_and_closed_forms(orc):
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 50
+    for _ in range(40):
+        h = herm(rng.uniform(0.01, 6.0))
+        ex = mp.expm(mp.matrix(h.tolist()) * mp.mpc(0, 1))
+        ex = np.array([[complex(ex[i, j]) for j in range(3)] for i in range(3)])
+        assert np.abs(orc.su3_exponentiate(h, 1.0) - ex).max() < 1e-13
+    for th in (0.0, 0.3, 2.0):
+        e = orc.su3_exponentiate(th * L[2], 1.0)
+        assert np.abs(e - np.diag([np.exp(1j * th), np.exp(-1j * th), 1])).max() < 1e-15
+    u = orc.su3_exponentiate(herm(3.0), 1.0)
+    assert abs(np.linalg.det(u) - 1) < 1e-13
+    assert np.abs(u @ u.conj().T - np.eye(3)).max() < 1e-14
+
+
+# ---------------------------------------------------------------- FFT
+@pytest.mark.parametrize("N", [2, 4, 8, 16])
+def test_fft_bruteforce(orc, N):
+    f = rng.normal(size=(N * N, 3)) + 1j * rng.normal(size=(N * N, 3))
+    for d in (0, 1):
+        assert np.abs(orc.fft2d(f, N, d) - orc.dft2d_bruteforce(f, N, d)).max() < 1e-14
+
+
+@pytest.mark.parametrize("N", [4, 32, 64, 256])
+def test_fft_numpy_conventions(orc, N):
+    f = rng.normal(size=(N * N, 9)) + 1j * rng.normal(size=(N * N, 9))
+    for d in (0, 1):
+        assert np.abs(orc.fft2d(f, N, d) - npfft(f, N, d)).max() < 1e-13
+    # Parseval and round trip
+    g = orc.fft2d(f, N, 1)
+    assert abs(np.sum(abs(g) ** 2) / np.sum(abs(f) ** 2) - 1) < 1e-13
+    assert np.abs(orc.fft2d(g, N, 0) - f).max() < 1e-13
+
+
+def test_fft_delta_known_answer(orc):
+    # tests/test_fftw_execute.cpp:59-72: delta at the origin -> constant 1/sqrt(V)
+    N = 64
+    f = np.zeros((N * N, 1), complex)
+    f[0] = 1.0
+    g = orc.fft2d(f, N, 0)
+    assert np.abs(g - 1.0 / N).max() < 1e-16
+
+
+# ---------------------------------------------------------------- noise basis
+def test_lambda_basis(orc):
+    N = 4
+    z = rng.normal(size=N * N * 16)
+    xx, xy = orc.generate_gaussian(N, z)
+    for s in range(N * N):
+        for field, off in ((xx, 0), (xy, 1)):
+            m = field[s].reshape(3, 3)
+            n = z[16 * s + off::2][:8]
+            assert np.abs(m - np.einsum("a,aij->ij", n, LT)).max() < 1e-15
+            assert np.abs(m - m.conj().T).max() == 0.0
+            assert abs(np.trace(m)) < 1e-15
+            c = np.array([np.trace(m @ LT[a]).real / 2 for a in range(8)])
+            assert np.abs(c - n).max() < 1e-14
+
+
+def test_noise_coupling_scaling(orc):
+    # field.h:2273: sqrt(alpha) evaluated at phat2 of the POSITION index
+    N = 8
+    z = rng.normal(size=N * N * 16)
+    a, _ = orc.generate_gaussian(N, z)
+    b, _ = orc.generate_gaussian_noise_coupling(N, z)
+    ph, _, _ = orc.momenta(N)
+    q = ph * N * N / 36.0
+    alpha = 4 * np.pi / (9 * 0.2 * np.log(6.25 ** 5 + q ** 5))   # same function, other arrangement
+    assert np.allclose(b, a * np.sqrt(alpha)[:, None], rtol=1e-14, atol=0)
+    # alpha freezes at 4 pi / (9 ln 6.25) in the IR
+    assert abs(orc.alpha_kernel(0.0, N) - 4 * np.pi / (9 * np.log(6.25))) < 1e-15
+
+
+# ---------------------------------------------------------------- kernel multiply
+@pytest.mark.parametrize("coupling", [O_mod.SQRT, O_mod.NOISE, O_mod.FIXED])
+@pytest.mark.parametrize("kernel", [O_mod.LINEAR, O_mod.SIN])
+def test_prepare_A_local_closed_form(orc, coupling, kernel):
+    N = 16
+    fx = rng.normal(size=(N * N, 9)) + 1j * rng.normal(size=(N * N, 9))
+    fy = rng.normal(size=(N * N, 9)) + 1j * rng.normal(size=(N * N, 9))
+    A = orc.prepare_A_local(fx, fy, N, coupling, kernel)
+    k = np.fft.fftfreq(N, 1.0 / N)               # wrapped mode numbers
+    KX, KY = np.meshgrid(k, k, indexing="ij")
+    if kernel == O_mod.SIN:
+        q2 = 4 * np.sin(np.pi * np.arange(N) / N)[:, None] ** 2 + 4 * np.sin(np.pi * np.arange(N) / N)[None, :] ** 2
+        bx = np.sin(2 * np.pi * np.arange(N) / N)[:, None] * np.ones((1, N))
+        by = np.sin(2 * np.pi * np.arange(N) / N)[None, :] * np.ones((N, 1))
+    else:
+        px, py = 2 * np.pi * KX / N, 2 * np.pi * KY / N
+        q2, bx, by = px ** 2 + py ** 2, px, py
+    c = 4 * np.pi / (9 * 0.2 * np.log(6.25 ** 5 + (q2 * N * N / 36) ** 5)) if coupling == O_mod.SQRT else 1.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        kx = -2j * np.pi * np.sqrt(c) * bx / q2
+        ky = -2j * np.pi * np.sqrt(c) * by / q2
+    kx[0, 0] = ky[0, 0] = 0
+    expect = kx.reshape(-1, 1) * fx + ky.reshape(-1, 1) * fy
+    assert np.allclose(A, expect, rtol=1e-13, atol=1e-13)
+    assert np.all(A[0] == 0)
+
+
+def test_prepare_A_local_rejects_hatta(orc):
+    N = 4
+    f = np.zeros((N * N, 9), complex)
+    with pytest.raises(ValueError):
+        orc.prepare_A_local(f, f, N, O_mod.HATTA, O_mod.SIN)
+
+
+# ---------------------------------------------------------------- oracle vs reference build, N = 64
+def test_momenta_match_reference_build(orc, ref):
+    for a, b in zip(orc.momenta(N64), ref.momenta()):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("coupling", [O_mod.SQRT, O_mod.NOISE])
+@pytest.mark.parametrize("kernel", [O_mod.LINEAR, O_mod.SIN])
+def test_prepare_A_local_matches_reference_build(orc, ref, coupling, kernel):
+    fx = rng.normal(size=(N64 * N64, 9)) + 1j * rng.normal(size=(N64 * N64, 9))
+    fy = rng.normal(size=(N64 * N64, 9)) + 1j * rng.normal(size=(N64 * N64, 9))
+    assert np.array_equal(orc.prepare_A_local(fx, fy, N64, coupling, kernel),
+                          ref.prepare_A_local(fx, fy, coupling, kernel))
+
+
+def test_site_functions_match_reference_build(orc, ref):
+    N = N64
+    U = random_su3_field(orc, N)
+    z = rng.normal(size=N * N * 16)
+    xx, xy = orc.generate_gaussian(N, z)
+    vo = orc.uxiulocal(U, xx, xy, N)
+    vr = ref.uxiulocal(U, xx, xy)
+    assert np.array_equal(vo[0], vr[0]) and np.array_equal(vo[1], vr[1])
+    A = npfft(orc.prepare_A_local(npfft(xx, N, 1), npfft(xy, N, 1), N, 1, 1), N, 0)
+    B = npfft(orc.prepare_A_local(npfft(vo[0], N, 1), npfft(vo[1], N, 1), N, 1, 1), N, 0)
+    assert np.abs(orc.update_uf(U, B, A, 1e-4, N) - ref.update_uf(U, B, A, 1e-4)).max() <= 1e-15
+    assert np.abs(orc.mul_exp(U, A, N) - ref.mul_exp(U, A)).max() <= 1e-15
+    assert np.array_equal(orc.set_unit(N), ref.set_unit())
+    f = npfft(orc.set_mv_model(N, rng.normal(size=N * N * 8), 0.0035), N, 1)
+    for mass in (1e-4, 0.3):
+        assert np.array_equal(orc.solve_poisson(f, N, mass, 1.0), ref.solve_poisson(f, mass, 1.0))
+    Uk = npfft(U, N, 1)
+    c = orc.trace(Uk, N)
+    assert np.array_equal(c, ref.trace(Uk))
+    s = orc.average_and_symmetrize(c, N)
+    assert np.array_equal(s, ref.average_and_symmetrize(c))
+    zero = np.zeros_like(s)
+    for a, b in zip(orc.reduce(s, zero, zero, N), ref.reduce(s, zero, zero)):
+        assert np.array_equal(a, b)
+
+
+def _step_composed_from_reference(ref, orc, U, z, coupling, step):
+    """main_optimized.cpp:243-304 composed from the reference's per-site code + numpy FFT."""
+    N = N64
+    if coupling == O_mod.NOISE:
+        xx, xy = orc.generate_gaussian_noise_coupling(N, z)
+    else:
+        xx, xy = orc.generate_gaussian(N, z)
+    fx, fy = npfft(xx, N, 1), npfft(xy, N, 1)
+    A = npfft(ref.prepare_A_local(fx, fy, coupling, O_mod.SIN), N, 0)
+    xx, xy = npfft(fx, N, 0), npfft(fy, N, 0)
+    vx, vy = ref.uxiulocal(U, xx, xy)
+    B = npfft(ref.prepare_A_local(npfft(vx, N, 1), npfft(vy, N, 1), coupling, O_mod.SIN), N, 0)
+    return ref.update_uf(U, B, A, step)
+
+
+@pytest.mark.parametrize("coupling", [O_mod.SQRT, O_mod.NOISE])
+def test_langevin_step_matches_reference_composition(orc, ref, coupling):
+    U = random_su3_field(orc, N64)
+    for _ in range(2):
+        z = rng.normal(size=N64 * N64 * 16)
+        a = orc.langevin_step(U, z, N64, coupling, O_mod.SIN, 1e-4)
+        b = _step_composed_from_reference(ref, orc, U, z, coupling, 1e-4)
+        # two correct FP64 FFTs (numpy's, the oracle's) differ at ~1e-16 relative;
+        # the 1/phat2 kernel lifts that to ~2e-13 per entry after one step at N = 64
+        assert np.abs(a - b).max() < 1e-12
+        U = a
+
+
+def test_mv_init_matches_reference_composition(orc, ref):
+    N, Ny, g, mu, mass = N64, 3, 1.0, 30.72 / 64, 0.5
+    z = rng.normal(size=Ny * N * N * 8)
+    Uo = orc.mv_init(N, z, Ny, g, mu, mass)
+    U = ref.set_unit()
+    for s in range(Ny):
+        f = orc.set_mv_model(N, z[s * N * N * 8:(s + 1) * N * N * 8], g * mu / np.sqrt(Ny))
+        f = npfft(ref.solve_poisson(npfft(f, N, 1), mass * g ** 2 * mu, g), N, 0)
+        U = ref.mul_exp(U, f)
+    assert np.abs(Uo - U).max() < 1e-12
+
+
+def test_measurement_matches_reference_composition(orc, ref):
+    U = random_su3_field(orc, N64)
+    c = orc.measure_corr(U, N64)
+    expect = ref.average_and_symmetrize(ref.trace(npfft(U, N64, 1)))
+    assert np.abs(c - expect).max() < 1e-17
+
+
+# ---------------------------------------------------------------- invariants of the step
+def test_step_invariants(orc):
+    N = 16
+    U = random_su3_field(orc, N)
+    # zero noise leaves U unchanged (A = B = 0, exp(0) = 1 exactly)
+    assert np.array_equal(orc.langevin_step(U, np.zeros(N * N * 16), N, O_mod.FIXED, O_mod.SIN, 1e-4), U)
+    U2 = orc.langevin_step(U, rng.normal(size=N * N * 16), N, O_mod.FIXED, O_mod.SIN, 1e-4)
+    m = U2.reshape(-1, 3, 3)
+    assert np.abs(np.einsum("sij,skj->sik", m, m.conj()) - np.eye(3)).max() < 1e-13
+    assert np.abs(np.linalg.det(m) - 1).max() < 1e-13
+    assert np.abs(U2 - U).max() > 1e-3
+
+
+# ---------------------------------------------------------------- dipole / Wiener-Khinchin
+def test_dipole_wiener_khinchin(orc):
+    N = 8
+    U = random_su3_field(orc, N)
+    corr = orc.measure_corr(U, N)
+    S_k = orc.dipole_from_corr_bruteforce(corr, N)
+    S_x = orc.dipole_position_space(U, N)
+    S_x_sym = np.zeros_like(S_x).reshape(N, N)
+    Sx = S_x.reshape(N, N)
+    for x in range(N):
+        for y in range(N):
+            t = [Sx[x, y], Sx[y, x]]
+            t = t + [Sx[(-x) % N, y], Sx[y, (-x) % N], Sx[x, (-y) % N], Sx[(-y) % N, x],
+                     Sx[(-x) % N, (-y) % N], Sx[(-y) % N, (-x) % N]]
+            S_x_sym[x, y] = np.mean(t)
+    assert np.abs(S_k - S_x_sym.reshape(-1)).max() < 1e-13
+    assert abs(S_k[0] - 1) < 1e-13
+    assert abs(corr[:, 0].real.sum() - 3.0 / (N * N)) < 1e-16
+
+
+def test_radial_bins(orc):
+    N = 8
+    S = np.arange(N * N, dtype=float)
+    r, s, c = orc.radial_bins(S, N, 6)
+    assert c[0] == 1 and s[0] == 0 and r[0] == 0
+    assert c.sum() == sum(1 for x in range(N) for y in range(N)
+                          if np.hypot(min(x, N - x) if x != N // 2 else N // 2,
+                                      min(y, N - y) if y != N // 2 else N // 2) < 6)
+
+
+# ---------------------------------------------------------------- Philox
+def test_philox_known_answers(orc):
+    # Random123 known-answer vectors for philox4x32-10 (Salmon et al., SC'11)
+    assert orc.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert orc.philox4x32_10([0xffffffff] * 4, [0xffffffff] * 2) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert orc.philox4x32_10([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0]) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_philox_normals_statistics(orc):
+    N = 64
+    z = orc.philox_normals(N, 7, 0, 0, 0x4C414E47, 16)
+    assert abs(z.mean()) < 0.02 and abs(z.std() - 1) < 0.02
+    z2 = orc.philox_normals(N, 7, 0, 1, 0x4C414E47, 16)
+    assert abs(np.corrcoef(z, z2)[0, 1]) < 0.02
