@@ -1,0 +1,178 @@
+/* jimwlk.h -- C ABI of the MI355X JIMWLK Langevin library (libjimwlk_hip.so).
+ *
+ * The reference (ElsevierSoftwareX/SOFTX-D-21-00092) has no FFI: its boundary is
+ * the C++ class and free-function surface that main_optimized.cpp:159-355 calls.
+ * Each entry point below replaces one piece of that surface and cites it.  The
+ * C++ classes in softx-d-21-00092_amd/host/ (lfield, gfield, fftw2D, momenta,
+ * MV_class, rand_class, positions, config, mpi_class) sit on top of this ABI.
+ *
+ * Conventions
+ *  - Lattice: N x N periodic, N a power of two in [4, 8192].  Nx == Ny as the
+ *    reference assumes (field.h:1471).
+ *  - Host fields are AoS arrays of complex doubles (interleaved re, im):
+ *    entry k = 3*row + col of site x*N + y at [(x*N + y)*t + k]  (field.h:199-209).
+ *    Device storage is opaque (SoA planes) and owned by the context.
+ *  - Every call returns JW_OK (0) or a negative JW_ERR_* code; jw_last_error()
+ *    returns the message of the last failure on the calling thread.  Calls are
+ *    asynchronous on the context's stream unless they return host data;
+ *    jw_sync() waits.  One host thread per context; not re-entrant.
+ *  - FP64 throughout (the reference's arithmetic).
+ */
+#ifndef JIMWLK_H
+#define JIMWLK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    JW_OK = 0,
+    JW_ERR_ARG = -1,          /* bad argument (size, enum, null pointer)          */
+    JW_ERR_HIP = -2,          /* HIP runtime error                                  */
+    JW_ERR_ALLOC = -3,        /* device allocation failed                           */
+    JW_ERR_UNSUPPORTED = -4,  /* valid in the reference but not supported here      */
+    JW_ERR_STATE = -5         /* call out of sequence (e.g. noise not provided)     */
+};
+
+/* Coupling choices (config.h:40, field.h:2022-2050) plus two additions:
+ *   JW_FIXED_COUPLING: kernel factor 1, white noise (the reference has no switch
+ *     for it on the momentum path);
+ *   JW_CORRELATED_NOISE_COUPLING: kernel factor 1, noise correlated per mode,
+ *     xi~ = F^-1[sqrt(alpha(k)) F xi] (the per-mode Cholesky factor of the
+ *     coupling covariance that field.h:1633-1651 + matrix.h:151-200 build densely).
+ * JW_HATTA_COUPLING_CONSTANT is rejected: the reference leaves the coupling
+ * uninitialised for it on this path (field.h:2022). */
+enum jw_coupling {
+    JW_SQRT_COUPLING_CONSTANT = 0,
+    JW_NOISE_COUPLING_CONSTANT = 1,
+    JW_HATTA_COUPLING_CONSTANT = 2,
+    JW_FIXED_COUPLING = 3,
+    JW_CORRELATED_NOISE_COUPLING = 4
+};
+
+/* Kernel choices (config.h:42) */
+enum jw_kernel { JW_LINEAR_KERNEL = 0, JW_SIN_KERNEL = 1 };
+
+/* Running coupling alpha(q2) = (4 pi / beta0) / ln[ (r^2)^(1/c) + (q2 N^2 / L^2)^(1/c) ]^c
+ * with L = lambda_L = Lambda*N, r = mu0/Lambda.  The reference hard-codes
+ * {6, 2.5, 0.2, 9} (field.h:2046, 2257-2273); a null pointer selects those. */
+typedef struct jw_alpha {
+    double lambda_L;
+    double mu0_over_lambda;
+    double freeze_c;
+    double beta0;
+} jw_alpha;
+
+typedef struct jw_ctx jw_ctx;
+typedef struct jw_field jw_field;
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the lattice set-up of main_optimized.cpp:64-92 for one rank:
+ * config sizes, momenta::set (momenta.h:90-125), fftw2D(config*) + init2D
+ * (mpi_fftw_class.h:82-189).  Multi-rank MPI is not part of this library:
+ * the context is one lattice on one GPU. */
+int jw_create(jw_ctx** out, int N, int device);
+void jw_destroy(jw_ctx* ctx);
+int jw_sync(jw_ctx* ctx);
+int jw_lattice_size(const jw_ctx* ctx);
+const char* jw_last_error(void);
+const char* jw_version(void);
+
+/* ---- fields: lfield<double,t>(Nxl, Nyl) (field.h:152-299) ------------------ */
+int jw_field_create(jw_ctx* ctx, int t, jw_field** out);          /* zero-filled (field.h:70-80) */
+void jw_field_destroy(jw_field* f);
+int jw_field_upload(jw_field* f, const double* host_aos);          /* 2*t*V doubles */
+int jw_field_download(const jw_field* f, double* host_aos);
+int jw_field_copy(jw_field* dst, const jw_field* src);             /* lfield::operator= (field.h:301-318) */
+int jw_set_to_unit(jw_field* f);                                   /* lfield::setToUnit (field.h:196-211) */
+int jw_set_to_zero(jw_field* f);                                   /* lfield::setToZero (field.h:180-188) */
+
+/* ---- noise --------------------------------------------------------------
+ * The reference seeds thread-local ranlux24 generators from the clock
+ * (field.h:692-697, 2131-2135) and cannot be reproduced.  Here the generators
+ * take either injected standard normals (one call's worth, in the reference's
+ * draw order: 16 per site n0,m0,n1,m1,... for the Langevin noise,
+ * field.h:2162-2165; 8 per site n0..n7 for an MV sheet, field.h:708-710), or
+ * the Philox4x32-10 stream: key {seed, traj}, counter {site, step, tag, call}.
+ * rand_class (rand_class.h:47-68) is re-backed by this stream. */
+int jw_noise_inject(jw_ctx* ctx, const double* host_normals, int per_site);
+int jw_noise_philox(jw_ctx* ctx, uint32_t seed, uint32_t traj, uint32_t step);
+int jw_noise_get_step(const jw_ctx* ctx, uint32_t* step);
+
+/* ---- reference per-function entry points (9-component fields) ------------ */
+/* fftw2D::execute2D (mpi_fftw_class.h:191-315): unitary, dir 1 = FFTW_FORWARD. */
+int jw_execute2D(jw_ctx* ctx, jw_field* f, int dir);
+/* generate_gaussian (field.h:2116-2243) */
+int jw_generate_gaussian(jw_ctx* ctx, jw_field* xi_x, jw_field* xi_y);
+/* generate_gaussian_with_noise_coupling_constant (field.h:2245-2365) */
+int jw_generate_gaussian_with_noise_coupling_constant(jw_ctx* ctx, jw_field* xi_x, jw_field* xi_y,
+                                                      const jw_alpha* alpha);
+/* prepare_A_local, 7-argument form (field.h:1998-2075) */
+int jw_prepare_A_local(jw_ctx* ctx, jw_field* A, const jw_field* xi_x, const jw_field* xi_y,
+                       int coupling, int kernel, const jw_alpha* alpha);
+/* uxiulocal (field.h:1892-1933) */
+int jw_uxiulocal(jw_ctx* ctx, jw_field* v_x, jw_field* v_y, const jw_field* U,
+                 const jw_field* xi_x, const jw_field* xi_y);
+/* update_uf (field.h:1966-1996) */
+int jw_update_uf(jw_ctx* ctx, jw_field* U, const jw_field* B, const jw_field* A, double step);
+/* lfield::setMVModel (field.h:683-757); mu is MV_class::mu_parameter */
+int jw_set_mv_model(jw_ctx* ctx, jw_field* f, double g, double mu, int Ny);
+/* lfield::solvePoisson (field.h:867-877) */
+int jw_solve_poisson(jw_ctx* ctx, jw_field* f, double mass, double g);
+/* lfield::operator*= : U <- U exp(i f) (field.h:337-365) */
+int jw_mul_exp(jw_ctx* ctx, jw_field* U, const jw_field* f);
+/* lfield::trace (field.h:1375-1432): corr (t = 1) = tr[(U/V)(U/V)^dagger] */
+int jw_trace(jw_ctx* ctx, const jw_field* U, jw_field* corr);
+/* gfield::average_and_symmetrize (field.h:1460-1492), in place on a t = 1 field */
+int jw_average_and_symmetrize(jw_ctx* ctx, jw_field* corr);
+/* gfield::reduce(sum, err, mpi) (field.h:1521-1536): sum += c, err += c^2 */
+int jw_reduce(jw_ctx* ctx, const jw_field* corr, jw_field* sum, jw_field* err);
+
+/* ---- evolution engine ----------------------------------------------------
+ * main_optimized.cpp:234-311, MOMENTUM_EVOLUTION branch: nsteps Langevin steps
+ * U <- exp(i sqrt(ds) B) U exp(-i sqrt(ds) A) (field.h:1966-1996), computed in
+ * the generator basis (8 real planes per su(3) field, two per complex FFT).
+ * Noise: the pending injection (nsteps must be 1) or the Philox stream (its
+ * step counter advances by one per step).  step = Delta s as the caller wants
+ * it (the reference parses it through float, config.h:92-93). */
+int jw_evolve(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
+              int nsteps);
+
+/* MV initial condition (main_optimized.cpp:178-191): U = 1, then Ny sheets
+ * U <- U exp(i A_s), A_s = F^-1[ F rho_s * (-g / (-phat2 + m^2)) ], rho_s with
+ * sigma = g mu / sqrt(Ny), m = mass * g^2 * mu.  Charges: injected (8*V*Ny
+ * normals, sheet-major; may be null to use Philox with the context's seed,
+ * traj and sheet index as the step) */
+int jw_mv_init(jw_ctx* ctx, jw_field* U, int Ny, double g, double mu, double mass,
+               const double* host_normals);
+
+/* ---- observables ----------------------------------------------------------
+ * Measurement (main_optimized.cpp:317-333): F U -> trace -> symmetrise, as a
+ * real array corr[V] (the reference's values are real; field.h:1375-1492). */
+int jw_measure_corr(jw_ctx* ctx, const jw_field* U, double* host_corr);
+/* Dipole S(r) = (V/3) Re sum_k corr(k) e^{i k.r} over the whole lattice
+ * (Wiener-Khinchin; the reference prints k-space only).  host_corr: V reals. */
+int jw_dipole_from_corr(jw_ctx* ctx, const double* host_corr, double* host_S);
+/* Radial bins of S(r) by minimal-image distance (bin b: b <= |r| < b+1) and the
+ * saturation scale Q_s = sqrt(2)/R_s, S(R_s) = exp(-1/2), linear interpolation.
+ * Host-side post-processing; *Qs = 0 when S never drops below exp(-1/2). */
+int jw_radial_bins(int N, const double* S, int nbins, double* rbar, double* Sbin, double* count);
+int jw_saturation_scale(int nbins, const double* rbar, const double* Sbin, const double* count, double* Qs);
+
+/* ---- instrumentation ------------------------------------------------------ */
+/* Enable per-kernel timing with HIP events on the context's stream. */
+int jw_timing(jw_ctx* ctx, int enable);
+/* Average duration (ms) and call count of kernel class `which` (see jw_kernel_name). */
+int jw_kernel_time(const jw_ctx* ctx, int which, double* avg_ms, long* calls);
+const char* jw_kernel_name(int which);
+/* exp(i s H(h)) for n generator vectors h (8 reals each) -> n 3x3 matrices (18 reals). */
+int jw_su3_exp_gen(jw_ctx* ctx, const double* host_h8, double s, double* host_out18, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JIMWLK_H */

@@ -1,0 +1,591 @@
+// jw_api.hip -- the C ABI declared in include/jimwlk.h.
+#include "jw_internal.hpp"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+using jw::cplx;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_CHECK(expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fail(JW_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+const jw::Alpha kAlphaRef = {6.0, 2.5, 0.2, 9.0};
+
+jw::Alpha to_alpha(const jw_alpha* a) {
+    if (!a) return kAlphaRef;
+    return jw::Alpha{a->lambda_L, a->mu0_over_lambda, a->freeze_c, a->beta0};
+}
+
+bool valid_N(int N) { return N >= 4 && N <= 8192 && (N & (N - 1)) == 0; }
+
+int ensure_ws(jw_ctx* c, size_t planes) {
+    if (c->ws_planes >= planes) return JW_OK;
+    if (c->ws) hipFree(c->ws);
+    c->ws = nullptr;
+    c->ws_planes = 0;
+    if (hipMalloc(&c->ws, planes * c->V * sizeof(cplx)) != hipSuccess)
+        return fail(JW_ERR_ALLOC, "workspace of %zu planes (%zu MB) failed", planes,
+                    planes * c->V * sizeof(cplx) >> 20);
+    c->ws_planes = planes;
+    return JW_OK;
+}
+
+int ensure_sa(jw_ctx* c, int variant, const jw::Alpha& a) {
+    if (!c->sa) {
+        if (hipMalloc(&c->sa, c->V * sizeof(double)) != hipSuccess)
+            return fail(JW_ERR_ALLOC, "coupling table allocation failed");
+        c->sa_variant = -1;
+    }
+    if (c->sa_variant == variant && memcmp(&c->sa_params, &a, sizeof a) == 0) return JW_OK;
+    if (!(a.lambda_L > 0) || !(a.mu0_over_lambda > 1.0) || !(a.freeze_c > 0) || !(a.beta0 > 0))
+        return fail(JW_ERR_ARG, "invalid running-coupling parameters");
+    HIP_CHECK(jw::build_sqrt_alpha(c, variant, a));
+    c->sa_variant = variant;
+    c->sa_params = a;
+    return JW_OK;
+}
+
+// noise source for the next generator call; advances the Philox step afterwards
+jw::NoiseSrc take_noise(jw_ctx* c, int per_site, uint32_t tag, bool* ok) {
+    jw::NoiseSrc ns{nullptr, c->seed, c->traj, c->step, tag};
+    *ok = true;
+    if (c->inj_per_site) {
+        if (c->inj_per_site != per_site) { *ok = false; return ns; }
+        ns.inj = c->inj;
+        c->inj_per_site = 0;
+    } else {
+        c->step += 1;
+    }
+    return ns;
+}
+
+int check_field(const jw_field* f, const jw_ctx* c, int t, const char* name) {
+    if (!f) return fail(JW_ERR_ARG, "%s is null", name);
+    if (c && f->ctx != c) return fail(JW_ERR_ARG, "%s belongs to another context", name);
+    if (t && f->t != t) return fail(JW_ERR_ARG, "%s has %d components, expected %d", name, f->t, t);
+    return JW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* jw_last_error(void) { return g_err; }
+const char* jw_version(void) { return "jimwlk-mi355x 0.1 (gfx950, FP64)"; }
+
+int jw_create(jw_ctx** out, int N, int device) {
+    if (!out) return fail(JW_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!valid_N(N)) return fail(JW_ERR_ARG, "N = %d: must be a power of two in [4, 8192]", N);
+    int ndev = 0;
+    HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(JW_ERR_ARG, "device %d of %d", device, ndev);
+    HIP_CHECK(hipSetDevice(device));
+    jw_ctx* c = new jw_ctx();
+    memset(c, 0, sizeof *c);
+    c->N = N;
+    c->V = (size_t)N * N;
+    c->device = device;
+    c->sa_variant = -1;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(JW_ERR_HIP, "stream creation failed");
+    }
+    // twiddles exp(-2 pi i m/N) rounded from long double
+    std::vector<cplx> tw(N);
+    for (int m = 0; m < N; m++) {
+        long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)N;
+        tw[m] = make_double2((double)cosl(ang), (double)sinl(ang));
+    }
+    // momenta.h:90-125 per axis: 4 pow(sin(pi k/N), 2) and sin(2 pi k/N)
+    std::vector<double> q4(N), sb(N);
+    for (int k = 0; k < N; k++) {
+        q4[k] = 4.0 * pow(sin(M_PI * k / (1.0 * N)), 2.0);
+        sb[k] = sin(2.0 * M_PI * k / (1.0 * N));
+    }
+    if (hipMalloc(&c->tw, N * sizeof(cplx)) != hipSuccess || hipMalloc(&c->q4, N * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->sb, N * sizeof(double)) != hipSuccess) {
+        jw_destroy(c);
+        return fail(JW_ERR_ALLOC, "table allocation failed");
+    }
+    hipMemcpy(c->tw, tw.data(), N * sizeof(cplx), hipMemcpyHostToDevice);
+    hipMemcpy(c->q4, q4.data(), N * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(c->sb, sb.data(), N * sizeof(double), hipMemcpyHostToDevice);
+    *out = c;
+    return JW_OK;
+}
+
+void jw_destroy(jw_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->tw) hipFree(c->tw);
+    if (c->q4) hipFree(c->q4);
+    if (c->sb) hipFree(c->sb);
+    if (c->sa) hipFree(c->sa);
+    if (c->inj) hipFree(c->inj);
+    if (c->ws) hipFree(c->ws);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int jw_sync(jw_ctx* c) {
+    if (!c) return fail(JW_ERR_ARG, "null context");
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return JW_OK;
+}
+
+int jw_lattice_size(const jw_ctx* c) { return c ? c->N : 0; }
+
+// ---------------------------------------------------------------- fields
+int jw_field_create(jw_ctx* c, int t, jw_field** out) {
+    if (!c || !out) return fail(JW_ERR_ARG, "null argument");
+    if (t != 1 && t != 9) return fail(JW_ERR_ARG, "t = %d: fields have 1 or 9 components", t);
+    jw_field* f = new jw_field{c, t, nullptr};
+    if (hipMalloc(&f->d, t * c->V * sizeof(cplx)) != hipSuccess) {
+        delete f;
+        return fail(JW_ERR_ALLOC, "field allocation (%zu MB) failed", t * c->V * sizeof(cplx) >> 20);
+    }
+    HIP_CHECK(hipMemsetAsync(f->d, 0, t * c->V * sizeof(cplx), c->stream));
+    *out = f;
+    return JW_OK;
+}
+
+void jw_field_destroy(jw_field* f) {
+    if (!f) return;
+    hipStreamSynchronize(f->ctx->stream);
+    hipFree(f->d);
+    delete f;
+}
+
+int jw_field_upload(jw_field* f, const double* h) {
+    if (!f || !h) return fail(JW_ERR_ARG, "null argument");
+    jw_ctx* c = f->ctx;
+    const size_t V = c->V;
+    std::vector<cplx> soa(f->t * V);
+    for (size_t i = 0; i < V; i++)
+        for (int k = 0; k < f->t; k++) soa[k * V + i] = make_double2(h[2 * (i * f->t + k)], h[2 * (i * f->t + k) + 1]);
+    HIP_CHECK(hipMemcpyAsync(f->d, soa.data(), soa.size() * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return JW_OK;
+}
+
+int jw_field_download(const jw_field* f, double* h) {
+    if (!f || !h) return fail(JW_ERR_ARG, "null argument");
+    jw_ctx* c = f->ctx;
+    const size_t V = c->V;
+    std::vector<cplx> soa(f->t * V);
+    HIP_CHECK(hipMemcpyAsync(soa.data(), f->d, soa.size() * sizeof(cplx), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < V; i++)
+        for (int k = 0; k < f->t; k++) {
+            h[2 * (i * f->t + k)] = soa[k * V + i].x;
+            h[2 * (i * f->t + k) + 1] = soa[k * V + i].y;
+        }
+    return JW_OK;
+}
+
+int jw_field_copy(jw_field* dst, const jw_field* src) {
+    if (!dst || !src || dst->ctx != src->ctx || dst->t != src->t) return fail(JW_ERR_ARG, "incompatible fields");
+    if (dst == src) return JW_OK;
+    HIP_CHECK(hipMemcpyAsync(dst->d, src->d, dst->t * dst->ctx->V * sizeof(cplx), hipMemcpyDeviceToDevice,
+                             dst->ctx->stream));
+    return JW_OK;
+}
+
+int jw_set_to_unit(jw_field* f) {
+    if (int e = check_field(f, nullptr, 9, "field")) return e;
+    HIP_CHECK(jw::set_unit(f->ctx, f->d));
+    return JW_OK;
+}
+
+int jw_set_to_zero(jw_field* f) {
+    if (!f) return fail(JW_ERR_ARG, "null field");
+    HIP_CHECK(hipMemsetAsync(f->d, 0, f->t * f->ctx->V * sizeof(cplx), f->ctx->stream));
+    return JW_OK;
+}
+
+// ---------------------------------------------------------------- noise
+int jw_noise_inject(jw_ctx* c, const double* h, int per_site) {
+    if (!c || !h) return fail(JW_ERR_ARG, "null argument");
+    if (per_site != 16 && per_site != 8) return fail(JW_ERR_ARG, "per_site = %d: 16 (Langevin) or 8 (MV)", per_site);
+    if (!c->inj && hipMalloc(&c->inj, 16 * c->V * sizeof(double)) != hipSuccess)
+        return fail(JW_ERR_ALLOC, "noise buffer allocation failed");
+    HIP_CHECK(hipMemcpyAsync(c->inj, h, per_site * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->inj_per_site = per_site;
+    return JW_OK;
+}
+
+int jw_noise_philox(jw_ctx* c, uint32_t seed, uint32_t traj, uint32_t step) {
+    if (!c) return fail(JW_ERR_ARG, "null context");
+    c->seed = seed;
+    c->traj = traj;
+    c->step = step;
+    c->inj_per_site = 0;
+    return JW_OK;
+}
+
+int jw_noise_get_step(const jw_ctx* c, uint32_t* step) {
+    if (!c || !step) return fail(JW_ERR_ARG, "null argument");
+    *step = c->step;
+    return JW_OK;
+}
+
+// ---------------------------------------------------------------- per-function entry points
+int jw_execute2D(jw_ctx* c, jw_field* f, int dir) {
+    if (int e = check_field(f, c, 0, "field")) return e;
+    HIP_CHECK(jw::fft2d(c, f->d, f->t, dir ? 1 : 0, 1.0 / c->N));
+    return JW_OK;
+}
+
+static int generate_common(jw_ctx* c, jw_field* x, jw_field* y, const double* sa_pos) {
+    bool ok;
+    jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
+    if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
+    HIP_CHECK(jw::generate_gaussian(c, x->d, y->d, ns, sa_pos));
+    return JW_OK;
+}
+
+int jw_generate_gaussian(jw_ctx* c, jw_field* x, jw_field* y) {
+    if (int e = check_field(x, c, 9, "xi_x")) return e;
+    if (int e = check_field(y, c, 9, "xi_y")) return e;
+    return generate_common(c, x, y, nullptr);
+}
+
+int jw_generate_gaussian_with_noise_coupling_constant(jw_ctx* c, jw_field* x, jw_field* y, const jw_alpha* a) {
+    if (int e = check_field(x, c, 9, "xi_x")) return e;
+    if (int e = check_field(y, c, 9, "xi_y")) return e;
+    if (int e = ensure_sa(c, 1, to_alpha(a))) return e;
+    return generate_common(c, x, y, c->sa);
+}
+
+int jw_prepare_A_local(jw_ctx* c, jw_field* A, const jw_field* x, const jw_field* y, int coupling, int kernel,
+                       const jw_alpha* a) {
+    if (int e = check_field(A, c, 9, "A")) return e;
+    if (int e = check_field(x, c, 9, "xi_x")) return e;
+    if (int e = check_field(y, c, 9, "xi_y")) return e;
+    if (coupling == JW_HATTA_COUPLING_CONSTANT)
+        return fail(JW_ERR_UNSUPPORTED, "HATTA coupling: the reference leaves the coupling uninitialised here");
+    if (coupling < 0 || coupling > 4 || (kernel != 0 && kernel != 1)) return fail(JW_ERR_ARG, "bad coupling/kernel");
+    const double* sa = nullptr;
+    if (coupling == JW_SQRT_COUPLING_CONSTANT && kernel == JW_SIN_KERNEL) {
+        if (int e = ensure_sa(c, 0, to_alpha(a))) return e;
+        sa = c->sa;
+    }
+    HIP_CHECK(jw::prepare_A_local(c, A->d, x->d, y->d, coupling == 0 ? 0 : 1, kernel, sa, to_alpha(a)));
+    return JW_OK;
+}
+
+int jw_uxiulocal(jw_ctx* c, jw_field* vx, jw_field* vy, const jw_field* U, const jw_field* x, const jw_field* y) {
+    if (int e = check_field(vx, c, 9, "v_x")) return e;
+    if (int e = check_field(vy, c, 9, "v_y")) return e;
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (int e = check_field(x, c, 9, "xi_x")) return e;
+    if (int e = check_field(y, c, 9, "xi_y")) return e;
+    HIP_CHECK(jw::uxiulocal(c, vx->d, vy->d, U->d, x->d, y->d));
+    return JW_OK;
+}
+
+int jw_update_uf(jw_ctx* c, jw_field* U, const jw_field* B, const jw_field* A, double step) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (int e = check_field(B, c, 9, "B")) return e;
+    if (int e = check_field(A, c, 9, "A")) return e;
+    if (!(step >= 0)) return fail(JW_ERR_ARG, "step must be >= 0");
+    HIP_CHECK(jw::update_uf(c, U->d, B->d, A->d, step));
+    return JW_OK;
+}
+
+int jw_set_mv_model(jw_ctx* c, jw_field* f, double g, double mu, int Ny) {
+    if (int e = check_field(f, c, 9, "f")) return e;
+    if (Ny < 1) return fail(JW_ERR_ARG, "Ny must be >= 1");
+    bool ok;
+    jw::NoiseSrc ns = take_noise(c, 8, JW_TAG_MV, &ok);
+    if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 8", c->inj_per_site);
+    HIP_CHECK(jw::set_mv_model(c, f->d, ns, g * mu / sqrt((double)Ny)));
+    return JW_OK;
+}
+
+int jw_solve_poisson(jw_ctx* c, jw_field* f, double mass, double g) {
+    if (int e = check_field(f, c, 0, "f")) return e;
+    HIP_CHECK(jw::solve_poisson(c, f->d, f->t, mass, g));
+    return JW_OK;
+}
+
+int jw_mul_exp(jw_ctx* c, jw_field* U, const jw_field* f) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (int e = check_field(f, c, 9, "f")) return e;
+    HIP_CHECK(jw::mul_exp(c, U->d, f->d));
+    return JW_OK;
+}
+
+int jw_trace(jw_ctx* c, const jw_field* U, jw_field* corr) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (int e = check_field(corr, c, 1, "corr")) return e;
+    HIP_CHECK(jw::trace(c, U->d, corr->d));
+    return JW_OK;
+}
+
+int jw_average_and_symmetrize(jw_ctx* c, jw_field* corr) {
+    if (int e = check_field(corr, c, 1, "corr")) return e;
+    if (int e = ensure_ws(c, 24)) return e;
+    HIP_CHECK(jw::average_and_symmetrize(c, corr->d, c->ws));
+    HIP_CHECK(hipMemcpyAsync(corr->d, c->ws, c->V * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+    return JW_OK;
+}
+
+int jw_reduce(jw_ctx* c, const jw_field* corr, jw_field* sum, jw_field* err) {
+    if (int e = check_field(corr, c, 1, "corr")) return e;
+    if (int e = check_field(sum, c, 1, "sum")) return e;
+    if (int e = check_field(err, c, 1, "err")) return e;
+    HIP_CHECK(jw::reduce(c, corr->d, sum->d, err->d));
+    return JW_OK;
+}
+
+// ---------------------------------------------------------------- evolution
+// 9-component form of main_optimized.cpp:243-304 (used for the LINEAR kernel,
+// whose Nyquist line is not odd in k and so does not preserve the real
+// generator planes that the packed form relies on)
+static int evolve_9comp(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, const jw_alpha* a) {
+    if (coupling == JW_CORRELATED_NOISE_COUPLING)
+        return fail(JW_ERR_UNSUPPORTED, "LINEAR kernel with correlated noise is not supported");
+    if (int e = ensure_ws(c, 54)) return e;
+    cplx* xx = c->ws;
+    cplx* xy = xx + 9 * c->V;
+    cplx* A = xy + 9 * c->V;
+    cplx* vx = A + 9 * c->V;
+    cplx* vy = vx + 9 * c->V;
+    cplx* B = vy + 9 * c->V;
+    const double* sa_pos = nullptr;
+    if (coupling == JW_NOISE_COUPLING_CONSTANT) {
+        if (int e = ensure_sa(c, 1, to_alpha(a))) return e;
+        sa_pos = c->sa;
+    }
+    bool ok;
+    jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
+    if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
+    const double s = 1.0 / c->N;
+    HIP_CHECK(jw::generate_gaussian(c, xx, xy, ns, sa_pos));
+    HIP_CHECK(jw::fft2d(c, xx, 18, 1, s));                       // xi_x, xi_y are adjacent
+    const int kc = coupling == JW_SQRT_COUPLING_CONSTANT ? 0 : 1;
+    HIP_CHECK(jw::prepare_A_local(c, A, xx, xy, kc, kernel, nullptr, to_alpha(a)));
+    HIP_CHECK(jw::fft2d(c, A, 9, 0, s));
+    HIP_CHECK(jw::fft2d(c, xx, 18, 0, s));
+    HIP_CHECK(jw::uxiulocal(c, vx, vy, U->d, xx, xy));
+    HIP_CHECK(jw::fft2d(c, vx, 18, 1, s));
+    HIP_CHECK(jw::prepare_A_local(c, B, vx, vy, kc, kernel, nullptr, to_alpha(a)));
+    HIP_CHECK(jw::fft2d(c, B, 9, 0, s));
+    HIP_CHECK(jw::update_uf(c, U->d, B, A, step));
+    return JW_OK;
+}
+
+int jw_evolve(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, const jw_alpha* a, int nsteps) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (coupling == JW_HATTA_COUPLING_CONSTANT)
+        return fail(JW_ERR_UNSUPPORTED, "HATTA coupling: the reference leaves the coupling uninitialised here");
+    if (coupling < 0 || coupling > 4 || (kernel != 0 && kernel != 1)) return fail(JW_ERR_ARG, "bad coupling/kernel");
+    if (!(step >= 0)) return fail(JW_ERR_ARG, "step must be >= 0");
+    if (nsteps < 0) return fail(JW_ERR_ARG, "nsteps < 0");
+    if (c->inj_per_site && nsteps > 1) return fail(JW_ERR_STATE, "an injection covers one step; nsteps = %d", nsteps);
+    if (kernel == JW_LINEAR_KERNEL) {
+        for (int s = 0; s < nsteps; s++)
+            if (int e = evolve_9comp(c, U, coupling, kernel, step, a)) return e;
+        return JW_OK;
+    }
+    if (int e = ensure_ws(c, 24)) return e;
+    const size_t V = c->V;
+    cplx* Xi = c->ws;
+    cplx* Vv = Xi + 8 * V;
+    cplx* ZA = Vv + 8 * V;
+    cplx* ZB = ZA + 4 * V;
+    const double* sa_pos = nullptr;
+    const double* sa_k = nullptr;
+    if (coupling == JW_NOISE_COUPLING_CONSTANT) {
+        if (int e = ensure_sa(c, 1, to_alpha(a))) return e;
+        sa_pos = c->sa;
+    } else if (coupling == JW_SQRT_COUPLING_CONSTANT || coupling == JW_CORRELATED_NOISE_COUPLING) {
+        if (int e = ensure_sa(c, 0, to_alpha(a))) return e;
+        sa_k = c->sa;
+    }
+    const int kc = coupling == JW_SQRT_COUPLING_CONSTANT ? 0 : 1;
+    const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
+    const double s = 1.0 / c->N, ss = sqrt(step);
+    for (int it = 0; it < nsteps; it++) {
+        bool ok;
+        jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
+        if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
+        HIP_CHECK(jw::gen_noise_packed(c, Xi, ns, sa_pos));
+        HIP_CHECK(jw::fft2d(c, Xi, 8, 1, s));
+        HIP_CHECK(jw::kernel_mult_packed(c, ZA, Xi, Xi + 4 * V, kc, sa_k, corr));
+        HIP_CHECK(jw::fft2d(c, ZA, 4, 0, s));
+        if (corr) HIP_CHECK(jw::fft2d(c, Xi, 8, 0, s));
+        HIP_CHECK(jw::uxiu_packed(c, Vv, U->d, corr ? Xi : nullptr, ns, sa_pos));
+        HIP_CHECK(jw::fft2d(c, Vv, 8, 1, s));
+        HIP_CHECK(jw::kernel_mult_packed(c, ZB, Vv, Vv + 4 * V, kc, sa_k, 0));
+        HIP_CHECK(jw::fft2d(c, ZB, 4, 0, s));
+        HIP_CHECK(jw::update_packed(c, U->d, ZA, ZB, ss));
+    }
+    return JW_OK;
+}
+
+int jw_mv_init(jw_ctx* c, jw_field* U, int Ny, double g, double mu, double mass, const double* host) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (Ny < 1) return fail(JW_ERR_ARG, "Ny must be >= 1");
+    if (int e = ensure_ws(c, 24)) return e;
+    cplx* R = c->ws;
+    HIP_CHECK(jw::set_unit(c, U->d));
+    const double sigma = g * mu / sqrt((double)Ny);
+    const double m = mass * pow(g, 2.0) * mu;
+    for (int s = 0; s < Ny; s++) {
+        jw::NoiseSrc ns{nullptr, c->seed, c->traj, (uint32_t)s, JW_TAG_MV};
+        if (host) {
+            if (int e = jw_noise_inject(c, host + (size_t)s * 8 * c->V, 8)) return e;
+            ns.inj = c->inj;
+            c->inj_per_site = 0;
+        }
+        HIP_CHECK(jw::mv_noise_packed(c, R, ns, sigma));
+        HIP_CHECK(jw::fft2d(c, R, 4, 1, 1.0 / c->N));
+        HIP_CHECK(jw::poisson_packed(c, R, m, g));
+        HIP_CHECK(jw::fft2d(c, R, 4, 0, 1.0 / c->N));
+        HIP_CHECK(jw::mul_exp_packed(c, U->d, R));
+    }
+    return JW_OK;
+}
+
+// ---------------------------------------------------------------- observables
+int jw_measure_corr(jw_ctx* c, const jw_field* U, double* host_corr) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (!host_corr) return fail(JW_ERR_ARG, "null output");
+    if (int e = ensure_ws(c, 24)) return e;
+    const size_t V = c->V;
+    cplx* Uk = c->ws;
+    cplx* tr = Uk + 9 * V;
+    cplx* sym = tr + V;
+    HIP_CHECK(hipMemcpyAsync(Uk, U->d, 9 * V * sizeof(cplx), hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(jw::fft2d(c, Uk, 9, 1, 1.0 / c->N));
+    HIP_CHECK(jw::trace(c, Uk, tr));
+    HIP_CHECK(jw::average_and_symmetrize(c, tr, sym));
+    std::vector<cplx> h(V);
+    HIP_CHECK(hipMemcpyAsync(h.data(), sym, V * sizeof(cplx), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < V; i++) host_corr[i] = h[i].x;
+    return JW_OK;
+}
+
+int jw_dipole_from_corr(jw_ctx* c, const double* host_corr, double* host_S) {
+    if (!c || !host_corr || !host_S) return fail(JW_ERR_ARG, "null argument");
+    if (int e = ensure_ws(c, 24)) return e;
+    const size_t V = c->V;
+    std::vector<cplx> h(V);
+    for (size_t i = 0; i < V; i++) h[i] = make_double2(host_corr[i], 0.0);
+    HIP_CHECK(hipMemcpyAsync(c->ws, h.data(), V * sizeof(cplx), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(jw::fft2d(c, c->ws, 1, 0, 1.0));
+    HIP_CHECK(hipMemcpyAsync(h.data(), c->ws, V * sizeof(cplx), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const double f = (double)V / 3.0;
+    for (size_t i = 0; i < V; i++) host_S[i] = f * h[i].x;
+    return JW_OK;
+}
+
+int jw_radial_bins(int N, const double* S, int nbins, double* rbar, double* Sb, double* cnt) {
+    if (N < 1 || !S || nbins < 1 || !rbar || !Sb || !cnt) return fail(JW_ERR_ARG, "bad argument");
+    std::vector<long double> sr(nbins, 0), ss(nbins, 0), sc(nbins, 0);
+    for (int x = 0; x < N; x++)
+        for (int y = 0; y < N; y++) {
+            const int dx = x < N / 2 ? x : x - N, dy = y < N / 2 ? y : y - N;
+            const double r = sqrt((double)(dx * dx + dy * dy));
+            const int b = (int)floor(r);
+            if (b >= nbins) continue;
+            sr[b] += r;
+            ss[b] += S[(size_t)x * N + y];
+            sc[b] += 1;
+        }
+    for (int b = 0; b < nbins; b++) {
+        cnt[b] = (double)sc[b];
+        rbar[b] = sc[b] > 0 ? (double)(sr[b] / sc[b]) : 0.0;
+        Sb[b] = sc[b] > 0 ? (double)(ss[b] / sc[b]) : 0.0;
+    }
+    return JW_OK;
+}
+
+int jw_saturation_scale(int nbins, const double* rbar, const double* Sb, const double* cnt, double* Qs) {
+    if (nbins < 1 || !rbar || !Sb || !cnt || !Qs) return fail(JW_ERR_ARG, "bad argument");
+    const double target = exp(-0.5);
+    *Qs = 0.0;
+    int prev = -1;
+    for (int b = 0; b < nbins; b++) {
+        if (cnt[b] <= 0) continue;
+        if (Sb[b] <= target && prev >= 0) {
+            const double t = (Sb[prev] - target) / (Sb[prev] - Sb[b]);
+            const double Rs = rbar[prev] + t * (rbar[b] - rbar[prev]);
+            *Qs = Rs > 0 ? sqrt(2.0) / Rs : 0.0;
+            return JW_OK;
+        }
+        prev = b;
+    }
+    return JW_OK;
+}
+
+// ---------------------------------------------------------------- instrumentation
+static const char* kNames[] = {"fft_rows", "fft_cols", "gen_noise", "kernel_mult", "uxiu",
+                               "update", "mv", "observables", "reference_functions"};
+
+int jw_timing(jw_ctx* c, int enable) {
+    if (!c) return fail(JW_ERR_ARG, "null context");
+    c->timing = enable ? 1 : 0;
+    for (int k = 0; k < 16; k++) { c->kernel_ms[k] = 0; c->kernel_calls[k] = 0; }
+    return JW_OK;
+}
+
+int jw_kernel_time(const jw_ctx* c, int which, double* avg_ms, long* calls) {
+    if (!c || which < 0 || which >= (int)(sizeof kNames / sizeof kNames[0]) || !avg_ms || !calls)
+        return fail(JW_ERR_ARG, "bad argument");
+    *calls = c->kernel_calls[which];
+    *avg_ms = *calls ? c->kernel_ms[which] / *calls : 0.0;
+    return JW_OK;
+}
+
+const char* jw_kernel_name(int which) {
+    if (which < 0 || which >= (int)(sizeof kNames / sizeof kNames[0])) return "";
+    return kNames[which];
+}
+
+int jw_su3_exp_gen(jw_ctx* c, const double* h8, double s, double* out18, int n) {
+    if (!c || !h8 || !out18 || n < 0) return fail(JW_ERR_ARG, "bad argument");
+    if (n == 0) return JW_OK;
+    double *dh = nullptr, *dout = nullptr;
+    HIP_CHECK(hipMalloc(&dh, 8 * sizeof(double) * n));
+    if (hipMalloc(&dout, 18 * sizeof(double) * n) != hipSuccess) {
+        hipFree(dh);
+        return fail(JW_ERR_ALLOC, "allocation failed");
+    }
+    hipMemcpyAsync(dh, h8, 8 * sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+    hipError_t e = jw::exp_batch(c, dh, s, dout, n);
+    hipMemcpyAsync(out18, dout, 18 * sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    hipFree(dh);
+    hipFree(dout);
+    if (e != hipSuccess) return fail(JW_ERR_HIP, "exp kernel: %s", hipGetErrorString(e));
+    return JW_OK;
+}
+
+}  // extern "C"

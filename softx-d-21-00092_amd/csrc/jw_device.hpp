@@ -1,0 +1,308 @@
+// jw_device.hpp -- device-side building blocks of the MI355X JIMWLK solver:
+// complex FP64 arithmetic, SU(3) / su(3) algebra in the reference's colour basis,
+// the SU(3) exponential (closed-form eigenvalues + Newton divided differences),
+// the Philox4x32-10 noise stream and the LDS Stockham FFT.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace jw {
+
+// ---------------------------------------------------------------------------
+// complex double
+// ---------------------------------------------------------------------------
+typedef double2 cplx;
+
+__device__ __forceinline__ cplx mk(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cplx cmulc(cplx a, cplx b) { return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }  // a*conj(b)
+__device__ __forceinline__ cplx conjg(cplx a) { return mk(a.x, -a.y); }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return mk(a.x * s, a.y * s); }
+__device__ __forceinline__ cplx cmuli(cplx a) { return mk(-a.y, a.x); }       // i*a
+__device__ __forceinline__ cplx cmulmi(cplx a) { return mk(a.y, -a.x); }      // -i*a
+
+// ---------------------------------------------------------------------------
+// 3x3 complex matrices, row-major (entry k = 3*row + col, as field.h:199-209)
+// ---------------------------------------------------------------------------
+struct m3 { cplx m[9]; };
+
+__device__ __forceinline__ m3 mmul(const m3& a, const m3& b) {
+    m3 c;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplx s = cmul(a.m[3 * i], b.m[j]);
+            s = cadd(s, cmul(a.m[3 * i + 1], b.m[3 + j]));
+            s = cadd(s, cmul(a.m[3 * i + 2], b.m[6 + j]));
+            c.m[3 * i + j] = s;
+        }
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// su(3) generator coefficients in the reference's colour basis
+//   {l1, -l2, l3, l4, -l5, l6, -l7, l8}   (field.h:2169-2231)
+// h[0..7] <-> n[0..7]:  H01 = h0 + i h1, H02 = h3 + i h4, H12 = h5 + i h6,
+//                       H00 = h2 + h7/sqrt3, H11 = -h2 + h7/sqrt3, H22 = -2 h7/sqrt3.
+// ---------------------------------------------------------------------------
+#define JW_SQRT3 1.7320508075688772935274463415059
+#define JW_INV_SQRT3 0.57735026918962576450914878050196
+
+// Hermitian traceless matrix s*H(h)
+__device__ __forceinline__ m3 herm_from_gen(const double h[8], double s) {
+    m3 X;
+    const double d8 = h[7] * JW_INV_SQRT3;
+    X.m[0] = mk(s * (h[2] + d8), 0.0);
+    X.m[4] = mk(s * (-h[2] + d8), 0.0);
+    X.m[8] = mk(s * (-2.0 * d8), 0.0);
+    X.m[1] = mk(s * h[0], s * h[1]);
+    X.m[2] = mk(s * h[3], s * h[4]);
+    X.m[5] = mk(s * h[5], s * h[6]);
+    X.m[3] = conjg(X.m[1]);
+    X.m[6] = conjg(X.m[2]);
+    X.m[7] = conjg(X.m[5]);
+    return X;
+}
+
+// Generator coefficients of a (nearly) Hermitian matrix, read the way the
+// reference's exponential reads its argument: upper triangle and real diagonal
+// (su3_matrix.h:263-264).  The trace part of the diagonal is dropped.
+__device__ __forceinline__ void gen_from_upper(const m3& M, double h[8]) {
+    h[0] = M.m[1].x; h[1] = M.m[1].y;
+    h[3] = M.m[2].x; h[4] = M.m[2].y;
+    h[5] = M.m[5].x; h[6] = M.m[5].y;
+    h[2] = 0.5 * (M.m[0].x - M.m[4].x);
+    h[7] = (M.m[0].x + M.m[4].x - 2.0 * M.m[8].x) * (0.5 * JW_INV_SQRT3);
+}
+
+// ---------------------------------------------------------------------------
+// exp(i X) for X Hermitian traceless 3x3 (Cayley-Hamilton form).
+//   eigenvalues a >= b >= c of X from the depressed cubic l^3 - c1 l - c0 = 0,
+//   c1 = tr(X^2)/2, c0 = det X (trigonometric solution);
+//   exp(iX) = e^{ia} 1 + [a,b] (X - a) + [a,b,c] (X - a)(X - b)   (Newton form)
+//   with the divided differences of e^{ix}:
+//   [a,b] = i e^{i(a+b)/2} sinc((a-b)/2),  [a,b,c] = ([a,b]-[b,c])/(a-c).
+// All branches are smooth; X = 0 gives exactly 1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sinc_(double x) {
+    const double x2 = x * x;
+    if (fabs(x) < 1e-4) return 1.0 - x2 * (1.0 / 6.0) * (1.0 - x2 * (1.0 / 20.0));
+    return sin(x) / x;
+}
+
+__device__ __forceinline__ cplx expi(double t) {
+    double s, c;
+    sincos(t, &s, &c);
+    return mk(c, s);
+}
+
+// [a,b] of e^{ix}
+__device__ __forceinline__ cplx divdiff1(double a, double b) {
+    const cplx e = expi(0.5 * (a + b));
+    return cmuli(cscale(e, sinc_(0.5 * (a - b))));
+}
+
+// X given by generator coefficients h scaled by s; returns E = exp(i s H(h)).
+__device__ __forceinline__ m3 exp_i_gen(const double h[8], double s) {
+    m3 X = herm_from_gen(h, s);
+    double c1 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) c1 += h[a] * h[a];
+    c1 *= s * s;
+    m3 E;
+    if (!(c1 > 0.0)) {   // X == 0 (or NaN input: propagate through the sum below)
+#pragma unroll
+        for (int k = 0; k < 9; k++) E.m[k] = mk((k % 4 == 0) ? 1.0 : 0.0, 0.0);
+        if (c1 != c1) E.m[0] = mk(c1, c1);
+        return E;
+    }
+    // det X (Hermitian: real)
+    const double d0 = X.m[0].x, d1 = X.m[4].x, d2 = X.m[8].x;
+    const cplx x = X.m[1], y = X.m[2], z = X.m[5];
+    const cplx xz = cmul(x, z);
+    const double c0 = d0 * d1 * d2 + 2.0 * (xz.x * y.x + xz.y * y.y)
+                      - d0 * (z.x * z.x + z.y * z.y) - d1 * (y.x * y.x + y.y * y.y)
+                      - d2 * (x.x * x.x + x.y * x.y);
+    const double r = sqrt(c1 * (1.0 / 3.0));
+    const double r3x2 = 2.0 * r * r * r;
+    const double disc = fmax(0.0, (r3x2 - c0) * (r3x2 + c0));
+    const double th = atan2(sqrt(disc), c0) * (1.0 / 3.0);
+    double sth, cth;
+    sincos(th, &sth, &cth);
+    const double a = 2.0 * r * cth;
+    const double b = r * (-cth + JW_SQRT3 * sth);
+    const double c = r * (-cth - JW_SQRT3 * sth);
+    const cplx ea = expi(a);
+    const cplx dab = divdiff1(a, b);
+    cplx dabc;
+    if (a - c > 1e-3) {
+        const cplx dbc = divdiff1(b, c);
+        dabc = cscale(csub(dab, dbc), 1.0 / (a - c));
+    } else {
+        // all eigenvalues within 1e-3 of 0: second divided difference of e^{ix}
+        // by its Taylor series (a + b + c = 0 removes the first-order term)
+        dabc = mk(-0.5 + (a * a + b * b + c * c + a * b + b * c + c * a) * (1.0 / 24.0), 0.0);
+    }
+    // Y = X - a, Z = X - b; YZ Hermitian
+    m3 Y = X, Z = X;
+    Y.m[0].x -= a; Y.m[4].x -= a; Y.m[8].x -= a;
+    Z.m[0].x -= b; Z.m[4].x -= b; Z.m[8].x -= b;
+    const m3 YZ = mmul(Y, Z);
+#pragma unroll
+    for (int k = 0; k < 9; k++) E.m[k] = cadd(cmul(dab, Y.m[k]), cmul(dabc, YZ.m[k]));
+    E.m[0] = cadd(E.m[0], ea);
+    E.m[4] = cadd(E.m[4], ea);
+    E.m[8] = cadd(E.m[8], ea);
+    return E;
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and the library's noise
+// stream: key {seed, traj}, counter {site, step, tag, call}; each call gives
+// one Box-Muller pair from two 53-bit uniforms in (0,1).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+        const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ __forceinline__ void box_muller_pair(uint32_t site, uint32_t step, uint32_t tag, uint32_t call,
+                                                uint32_t seed, uint32_t traj, double& z0, double& z1) {
+    uint32_t c[4] = {site, step, tag, call};
+    philox4x32_10(c, seed, traj);
+    const uint64_t a = ((uint64_t)c[0] << 32) | c[1];
+    const uint64_t b = ((uint64_t)c[2] << 32) | c[3];
+    const double u0 = ((double)(a >> 11) + 0.5) * 0x1.0p-53;
+    const double u1 = ((double)(b >> 11) + 0.5) * 0x1.0p-53;
+    const double rr = sqrt(-2.0 * log(u0));
+    double s, co;
+    sincospi(2.0 * u1, &s, &co);
+    z0 = rr * co;
+    z1 = rr * s;
+}
+
+// ---------------------------------------------------------------------------
+// Radix-2/4/8 DFT kernels in registers.  DIR = -1: exp(-2 pi i jk/R) (forward),
+// DIR = +1: exp(+2 pi i jk/R) (backward).
+// ---------------------------------------------------------------------------
+template <int DIR>
+__device__ __forceinline__ cplx rot_q(cplx a) { return DIR < 0 ? cmulmi(a) : cmuli(a); }  // a * e^{DIR i pi/2}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft_small(cplx* v) {
+    if constexpr (R == 2) {
+        const cplx a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const cplx t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const cplx t2 = cadd(v[1], v[3]), t3 = rot_q<DIR>(csub(v[1], v[3]));
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    } else {   // R == 8: two radix-4 halves and the e^{DIR i pi k/4} twiddles
+        cplx e[4] = {v[0], v[2], v[4], v[6]};
+        cplx o[4] = {v[1], v[3], v[5], v[7]};
+        dft_small<4, DIR>(e);
+        dft_small<4, DIR>(o);
+        const double h = 0.70710678118654752440084436210485;
+        // o1 *= (1 + DIR i)/sqrt2 ; o2 *= DIR i ; o3 *= (-1 + DIR i)/sqrt2
+        const cplx o1 = DIR < 0 ? mk((o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h)
+                                : mk((o[1].x - o[1].y) * h, (o[1].y + o[1].x) * h);
+        const cplx o2 = rot_q<DIR>(o[2]);
+        const cplx o3 = DIR < 0 ? mk((-o[3].x + o[3].y) * h, (-o[3].y - o[3].x) * h)
+                                : mk((-o[3].x - o[3].y) * h, (-o[3].y + o[3].x) * h);
+        v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// In-LDS Stockham autosort FFT (mixed radix 8/4/2), out-of-order-free:
+// stage with radix R after a product P of earlier radices maps butterfly j
+// (k = j mod P) from inputs x[j + r N/R] to outputs x[(j-k) R + k + r P],
+// after twiddling input r by w_N^{r k N/(P R)}.  Every thread reads all its
+// inputs, the block synchronises, then writes, so one buffer suffices.
+// NSEQ sequences of length N; element n of sequence s at buf[n*ES + s*SS].
+// tw[m] = exp(-2 pi i m / N).  NT threads take part.
+// ---------------------------------------------------------------------------
+template <int REM>
+struct radix_of { static constexpr int value = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2); };
+
+template <int N, int P, int R, int NSEQ, int ES, int SS, int NT, int DIR>
+__device__ __forceinline__ void fft_stage(cplx* buf, const cplx* __restrict__ tw, int tid) {
+    constexpr int S = N / R;
+    constexpr int NB = S * NSEQ;
+    constexpr int PER = (NB + NT - 1) / NT;
+    cplx v[PER][R];
+    int jj[PER], ss[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = tid + q * NT;
+        int j, s;
+        if constexpr (ES == 1) { j = b % S; s = b / S; }       // rows: j fastest
+        else { s = b % NSEQ; j = b / NSEQ; }                    // column tiles: sequence fastest
+        jj[q] = j;
+        ss[q] = s;
+        if (NB % NT == 0 || b < NB) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q][r] = buf[(j + r * S) * ES + s * SS];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = tid + q * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int j = jj[q], s = ss[q];
+            const int k = j % P;
+            if constexpr (P > 1) {
+                constexpr int STRIDE = N / (P * R);
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    cplx w = tw[r * k * STRIDE];
+                    if (DIR > 0) w.y = -w.y;
+                    v[q][r] = cmul(v[q][r], w);
+                }
+            }
+            dft_small<R, DIR>(v[q]);
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[((j - k) * R + k + r * P) * ES + s * SS] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int P, int NSEQ, int ES, int SS, int NT, int DIR>
+struct fft_stages {
+    static __device__ __forceinline__ void run(cplx* buf, const cplx* __restrict__ tw, int tid) {
+        constexpr int R = radix_of<N / P>::value;
+        fft_stage<N, P, R, NSEQ, ES, SS, NT, DIR>(buf, tw, tid);
+        fft_stages<N, P * R, NSEQ, ES, SS, NT, DIR>::run(buf, tw, tid);
+    }
+};
+template <int N, int NSEQ, int ES, int SS, int NT, int DIR>
+struct fft_stages<N, N, NSEQ, ES, SS, NT, DIR> {
+    static __device__ __forceinline__ void run(cplx*, const cplx* __restrict__, int) {}
+};
+
+template <int N, int NSEQ, int ES, int SS, int NT, int DIR>
+__device__ __forceinline__ void lds_fft(cplx* buf, const cplx* __restrict__ tw, int tid) {
+    fft_stages<N, 1, NSEQ, ES, SS, NT, DIR>::run(buf, tw, tid);
+}
+
+}  // namespace jw

@@ -1,0 +1,106 @@
+// jw_internal.hpp -- context/field structures and kernel launchers shared by the
+// kernels (jw_kernels.hip) and the C ABI (jw_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/jimwlk.h"
+
+// Philox counter tags of the two noise streams
+#define JW_TAG_LANGEVIN 0x4C414E47u   // 'LANG'
+#define JW_TAG_MV 0x4D565348u         // 'MVSH'
+
+namespace jw {
+
+typedef double2 cplx;
+
+// Noise source for the generators: injected standard normals (AoS, `per_site`
+// per site, reference draw order) or the Philox stream.
+struct NoiseSrc {
+    const double* inj;     // device pointer or nullptr -> Philox
+    uint32_t seed, traj, step, tag;
+};
+
+struct Alpha {
+    double lambda_L, mu0_over_lambda, freeze_c, beta0;
+};
+
+}  // namespace jw
+
+struct jw_field {
+    jw_ctx* ctx;
+    int t;                 // components per site (1 or 9)
+    jw::cplx* d;           // t planes of V complex doubles (SoA)
+};
+
+struct jw_ctx {
+    int N;
+    size_t V;
+    int device;
+    hipStream_t stream;
+    jw::cplx* tw;          // tw[m] = exp(-2 pi i m / N), m < N
+    double* q4;            // q4[k] = 4 pow(sin(pi k / N), 2)   (momenta.h:111)
+    double* sb;            // sb[k] = sin(2 pi k / N)           (momenta.h:119-120)
+    // sqrt(alpha) table over the lattice, rebuilt when the variant/params change
+    double* sa;
+    int sa_variant;        // -1 none, 0 kernel formula, 1 noise formula
+    jw::Alpha sa_params;
+    // injected normals (AoS, up to 16 per site), valid for one generator call
+    double* inj;
+    int inj_per_site;      // 0 when no injection is pending
+    // Philox stream state for the generators
+    uint32_t seed, traj, step;
+    // workspace: generator-basis planes (packed) and scratch complex planes
+    jw::cplx* ws;
+    size_t ws_planes;
+    // per-kernel timing (HIP events on ctx->stream)
+    int timing;
+    double kernel_ms[16];
+    long kernel_calls[16];
+};
+
+namespace jw {
+
+// ---- launchers (all asynchronous on ctx->stream; return hipError_t) ----
+hipError_t fft2d(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale);
+hipError_t fft_rows(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale);
+hipError_t fft_cols(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale);
+
+hipError_t build_sqrt_alpha(jw_ctx* c, int variant, const Alpha& a);
+
+// reference per-function kernels on 9-component (SoA) fields
+hipError_t set_unit(jw_ctx* c, cplx* U);
+hipError_t generate_gaussian(jw_ctx* c, cplx* xix, cplx* xiy, const NoiseSrc& ns, const double* sa_pos);
+hipError_t prepare_A_local(jw_ctx* c, cplx* A, const cplx* xix, const cplx* xiy, int coupling, int kernel,
+                           const double* sa_k, const Alpha& al);
+hipError_t uxiulocal(jw_ctx* c, cplx* Vx, cplx* Vy, const cplx* U, const cplx* xix, const cplx* xiy);
+hipError_t update_uf(jw_ctx* c, cplx* U, const cplx* B, const cplx* A, double step);
+hipError_t set_mv_model(jw_ctx* c, cplx* f, const NoiseSrc& ns, double sigma);
+hipError_t solve_poisson(jw_ctx* c, cplx* f, int t, double mass, double g);
+hipError_t mul_exp(jw_ctx* c, cplx* U, const cplx* f);
+hipError_t trace(jw_ctx* c, const cplx* U, cplx* corr);
+hipError_t average_and_symmetrize(jw_ctx* c, const cplx* in, cplx* out);
+hipError_t reduce(jw_ctx* c, const cplx* corr, cplx* sum, cplx* err);
+hipError_t scale_planes(jw_ctx* c, cplx* p, size_t n, double s);
+
+// generator-basis (packed) engine kernels
+hipError_t gen_noise_packed(jw_ctx* c, cplx* Xi, const NoiseSrc& ns, const double* sa_pos);
+hipError_t kernel_mult_packed(jw_ctx* c, cplx* Z, const cplx* X, const cplx* Y, int coupling,
+                              const double* sa_k, int scale_inputs);
+hipError_t uxiu_packed(jw_ctx* c, cplx* Vv, const cplx* U, const cplx* Xi_pos, const NoiseSrc& ns,
+                       const double* sa_pos);
+hipError_t update_packed(jw_ctx* c, cplx* U, const cplx* ZA, const cplx* ZB, double sqrt_step);
+hipError_t mv_noise_packed(jw_ctx* c, cplx* R, const NoiseSrc& ns, double sigma);
+hipError_t poisson_packed(jw_ctx* c, cplx* R, double mass, double g);
+hipError_t mul_exp_packed(jw_ctx* c, cplx* U, const cplx* R);
+
+// observables
+hipError_t corr_from_Uk(jw_ctx* c, const cplx* Uk, double* corr);
+hipError_t accumulate_real(jw_ctx* c, const double* corr, double* sum, double* err);
+
+// unit-test hooks
+hipError_t exp_batch(jw_ctx* c, const double* h8, double s, double* out18, int n);
+
+}  // namespace jw

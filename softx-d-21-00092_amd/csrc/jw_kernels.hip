@@ -1,0 +1,760 @@
+// jw_kernels.hip -- HIP kernels of the MI355X JIMWLK solver (gfx950) and their
+// launchers.  Device storage: every field is a set of SoA planes of V complex
+// doubles; plane k of a 9-component field holds matrix entry k = 3*row + col;
+// site (x, y) sits at index x*N + y in every plane (field.h:199-209 order).
+//
+// Two families of kernels:
+//  * the reference's per-function operations on 9-component fields
+//    (generate_gaussian, prepare_A_local, uxiulocal, update_uf, ...).  Those
+//    whose arithmetic can be reproduced operation for operation are compiled
+//    without FMA contraction so that they agree with the reference to the bit;
+//  * the generator-basis engine: every su(3) field (xi_x, xi_y, A, V_x, V_y, B)
+//    is carried as 8 real generator planes packed two per complex plane, which
+//    turns the reference's 72 complex plane FFTs per step into 24.
+#include "jw_device.hpp"
+#include "jw_internal.hpp"
+
+#include <math.h>
+
+namespace jw {
+
+// ---------------------------------------------------------------------------
+// timing of kernel classes with HIP events on the context stream
+// ---------------------------------------------------------------------------
+enum KClass { K_FFT_ROWS = 0, K_FFT_COLS, K_GEN_NOISE, K_KMULT, K_UXIU, K_UPDATE, K_MV, K_OBS, K_DROPIN,
+              K_NCLASS };
+
+struct KTimer {
+    jw_ctx* c;
+    int cls;
+    hipEvent_t a, b;
+    KTimer(jw_ctx* c_, int cls_) : c(c_), cls(cls_) {
+        if (c->timing) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~KTimer() {
+        if (c->timing) {
+            hipEventRecord(b, c->stream);
+            hipEventSynchronize(b);
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, a, b);
+            c->kernel_ms[cls] += ms;
+            c->kernel_calls[cls] += 1;
+            hipEventDestroy(a);
+            hipEventDestroy(b);
+        }
+    }
+};
+
+static inline unsigned grid_for(size_t n, unsigned block) {
+    size_t g = (n + block - 1) / block;
+    if (g > 65535u * 16u) g = 65535u * 16u;
+    return (unsigned)(g ? g : 1);
+}
+
+#define GRID_STRIDE(i, n) for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); \
+                               i += (size_t)gridDim.x * blockDim.x)
+
+// non-contracting complex helpers for the bit-matching kernels
+__device__ __forceinline__ cplx xmul(cplx a, cplx b) {
+#pragma clang fp contract(off)
+    return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ cplx xadd(cplx a, cplx b) {
+#pragma clang fp contract(off)
+    return mk(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ m3 xmmul(const m3& a, const m3& b) {
+    m3 c;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplx s = mk(0.0, 0.0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) s = xadd(s, xmul(a.m[3 * i + k], b.m[3 * k + j]));
+            c.m[3 * i + j] = s;
+        }
+    return c;
+}
+
+// ===========================================================================
+// FFT passes
+// ===========================================================================
+// Rows per workgroup (at most N: a workgroup never reaches past its plane) and
+// columns per workgroup (at most N), each a divisor of N.
+template <int N>
+struct RowCfg {
+    static constexpr int RPW0 = (N >= 1024) ? 1 : (1024 / N);
+    static constexpr int RPW = RPW0 < N ? RPW0 : N;
+    static constexpr int ELEMS = RPW * N;
+    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 1024) ? 1024 : ELEMS / 8);
+    static_assert(RPW >= 1 && RPW <= N && N % RPW == 0, "row tiling must divide the plane");
+};
+template <int N>
+struct ColCfg {
+    static constexpr int W0 = (N >= 4096) ? 1 : ((4096 / N > 16) ? 16 : 4096 / N);
+    static constexpr int W = W0 < N ? W0 : N;
+    static constexpr int ELEMS = W * N;
+    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 1024) ? 1024 : ELEMS / 8);
+    static_assert(W >= 1 && W <= N && N % W == 0, "column tiling must divide the plane");
+};
+
+template <int N, int DIR>
+__global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_rows(cplx* __restrict__ data, size_t plane_stride,
+                                                             const cplx* __restrict__ tw, double scale) {
+    using C = RowCfg<N>;
+    __shared__ cplx lds[C::ELEMS];
+    const int tid = threadIdx.x;
+    cplx* base = data + blockIdx.y * plane_stride + (size_t)blockIdx.x * C::ELEMS;
+    for (int i = tid; i < C::ELEMS; i += C::NT) lds[i] = base[i];
+    __syncthreads();
+    lds_fft<N, C::RPW, 1, N, C::NT, DIR>(lds, tw, tid);
+    for (int i = tid; i < C::ELEMS; i += C::NT) base[i] = cscale(lds[i], scale);
+}
+
+template <int N, int DIR>
+__global__ __launch_bounds__(ColCfg<N>::NT) void k_fft_cols(cplx* __restrict__ data, size_t plane_stride,
+                                                             const cplx* __restrict__ tw, double scale) {
+    using C = ColCfg<N>;
+    __shared__ cplx lds[C::ELEMS];
+    const int tid = threadIdx.x;
+    cplx* base = data + blockIdx.y * plane_stride + (size_t)blockIdx.x * C::W;
+    for (int i = tid; i < C::ELEMS; i += C::NT) lds[i] = base[(size_t)(i / C::W) * N + (i % C::W)];
+    __syncthreads();
+    lds_fft<N, C::W, C::W, 1, C::NT, DIR>(lds, tw, tid);
+    for (int i = tid; i < C::ELEMS; i += C::NT) base[(size_t)(i / C::W) * N + (i % C::W)] = cscale(lds[i], scale);
+}
+
+template <int N>
+static hipError_t launch_rows(jw_ctx* c, cplx* p, int np, int dir, double scale) {
+    using C = RowCfg<N>;
+    dim3 grid(N / C::RPW, np);
+    KTimer t(c, K_FFT_ROWS);
+    if (dir) hipLaunchKernelGGL((k_fft_rows<N, -1>), grid, dim3(C::NT), 0, c->stream, p, c->V, c->tw, scale);
+    else hipLaunchKernelGGL((k_fft_rows<N, +1>), grid, dim3(C::NT), 0, c->stream, p, c->V, c->tw, scale);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_cols(jw_ctx* c, cplx* p, int np, int dir, double scale) {
+    using C = ColCfg<N>;
+    dim3 grid(N / C::W, np);
+    KTimer t(c, K_FFT_COLS);
+    if (dir) hipLaunchKernelGGL((k_fft_cols<N, -1>), grid, dim3(C::NT), 0, c->stream, p, c->V, c->tw, scale);
+    else hipLaunchKernelGGL((k_fft_cols<N, +1>), grid, dim3(C::NT), 0, c->stream, p, c->V, c->tw, scale);
+    return hipGetLastError();
+}
+
+#define JW_SIZE_SWITCH(N, CALL)                         \
+    switch (N) {                                        \
+        case 4: return CALL(4);                         \
+        case 8: return CALL(8);                         \
+        case 16: return CALL(16);                       \
+        case 32: return CALL(32);                       \
+        case 64: return CALL(64);                       \
+        case 128: return CALL(128);                     \
+        case 256: return CALL(256);                     \
+        case 512: return CALL(512);                     \
+        case 1024: return CALL(1024);                   \
+        case 2048: return CALL(2048);                   \
+        case 4096: return CALL(4096);                   \
+        case 8192: return CALL(8192);                   \
+        default: return hipErrorInvalidValue;           \
+    }
+
+hipError_t fft_rows(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale) {
+#define CALL_ROWS(n) launch_rows<n>(c, planes, nplanes, dir, scale)
+    JW_SIZE_SWITCH(c->N, CALL_ROWS)
+#undef CALL_ROWS
+}
+
+hipError_t fft_cols(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale) {
+#define CALL_COLS(n) launch_cols<n>(c, planes, nplanes, dir, scale)
+    JW_SIZE_SWITCH(c->N, CALL_COLS)
+#undef CALL_COLS
+}
+
+// unitary 2-D transform of nplanes planes: dir 1 -> exp(-i k.x), dir 0 -> exp(+i k.x),
+// scaled by `scale` once (1/N for the reference's 1/sqrt(V))
+hipError_t fft2d(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale) {
+    hipError_t e = fft_rows(c, planes, nplanes, dir, 1.0);
+    if (e != hipSuccess) return e;
+    return fft_cols(c, planes, nplanes, dir, scale);
+}
+
+// ===========================================================================
+// tables
+// ===========================================================================
+// variant 0: sqrt(alpha) with prepare_A_local's operation order (field.h:2045-2046)
+// variant 1: with generate_gaussian_with_noise_coupling_constant's (field.h:2257-2273)
+__global__ __launch_bounds__(256) void k_sqrt_alpha(double* __restrict__ sa, const double* __restrict__ q4, int N, int variant, Alpha a) {
+    const size_t V = (size_t)N * N;
+    GRID_STRIDE(i, V) {
+        const double ph = q4[i / N] + q4[i % N];
+        const double inner = pow(pow(a.mu0_over_lambda * a.mu0_over_lambda, 1.0 / a.freeze_c) +
+                                 pow((ph * N * N) / a.lambda_L / a.lambda_L, 1.0 / a.freeze_c), a.freeze_c);
+        double v;
+        if (variant == 0) v = sqrt(4.0 * M_PI / (a.beta0 * log(inner)));
+        else v = sqrt((4.0 * M_PI / a.beta0) / log(inner));
+        sa[i] = v;
+    }
+}
+
+hipError_t build_sqrt_alpha(jw_ctx* c, int variant, const Alpha& a) {
+    hipLaunchKernelGGL(k_sqrt_alpha, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, c->sa, c->q4, c->N,
+                       variant, a);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// noise
+// ===========================================================================
+__device__ __forceinline__ void normals16(const NoiseSrc& ns, size_t site, double z[16]) {
+    if (ns.inj) {
+        const double2* p = reinterpret_cast<const double2*>(ns.inj + 16 * site);
+#pragma unroll
+        for (int q = 0; q < 8; q++) { const double2 v = p[q]; z[2 * q] = v.x; z[2 * q + 1] = v.y; }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            box_muller_pair((uint32_t)site, ns.step, ns.tag, (uint32_t)q, ns.seed, ns.traj, z[2 * q], z[2 * q + 1]);
+    }
+}
+
+__device__ __forceinline__ void normals8(const NoiseSrc& ns, size_t site, double z[8]) {
+    if (ns.inj) {
+        const double2* p = reinterpret_cast<const double2*>(ns.inj + 8 * site);
+#pragma unroll
+        for (int q = 0; q < 4; q++) { const double2 v = p[q]; z[2 * q] = v.x; z[2 * q + 1] = v.y; }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            box_muller_pair((uint32_t)site, ns.step, ns.tag, (uint32_t)q, ns.seed, ns.traj, z[2 * q], z[2 * q + 1]);
+    }
+}
+
+// the reference's lambda-basis site map (field.h:2169-2231; setMVModel field.h:714-745),
+// operation for operation
+__device__ __forceinline__ void lambda_entries(const double n[8], cplx u[9]) {
+#pragma clang fp contract(off)
+    const double s3 = sqrt(3.0);
+    const double d = n[7] / s3;
+    u[0] = mk(n[2] + d, 0.0);
+    u[1] = mk(n[0], n[1]);
+    u[2] = mk(n[3], n[4]);
+    u[3] = mk(n[0], -n[1]);
+    u[4] = mk(-n[2] + d, 0.0);
+    u[5] = mk(n[5], n[6]);
+    u[6] = mk(n[3], -n[4]);
+    u[7] = mk(n[5], -n[6]);
+    u[8] = mk(-2.0 * n[7] / s3, 0.0);
+}
+
+// ===========================================================================
+// reference per-function kernels (9-component fields)
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_set_unit(cplx* __restrict__ U, size_t V) {
+    GRID_STRIDE(i, V) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) U[k * V + i] = mk((k == 0 || k == 4 || k == 8) ? 1.0 : 0.0, 0.0);
+    }
+}
+
+hipError_t set_unit(jw_ctx* c, cplx* U) {
+    KTimer t(c, K_DROPIN);
+    hipLaunchKernelGGL(k_set_unit, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, U, c->V);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_generate_gaussian(cplx* __restrict__ xx, cplx* __restrict__ xy, size_t V, NoiseSrc ns,
+                                    const double* __restrict__ sa_pos) {
+#pragma clang fp contract(off)
+    GRID_STRIDE(i, V) {
+        double z[16];
+        normals16(ns, i, z);
+        double n[8], m[8];
+        const double s = sa_pos ? sa_pos[i] : 1.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            n[k] = sa_pos ? z[2 * k] * s : z[2 * k];
+            m[k] = sa_pos ? z[2 * k + 1] * s : z[2 * k + 1];
+        }
+        cplx a[9], b[9];
+        lambda_entries(n, a);
+        lambda_entries(m, b);
+#pragma unroll
+        for (int k = 0; k < 9; k++) { xx[k * V + i] = a[k]; xy[k * V + i] = b[k]; }
+    }
+}
+
+hipError_t generate_gaussian(jw_ctx* c, cplx* xix, cplx* xiy, const NoiseSrc& ns, const double* sa_pos) {
+    KTimer t(c, K_DROPIN);
+    hipLaunchKernelGGL(k_generate_gaussian, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, xix, xiy, c->V,
+                       ns, sa_pos);
+    return hipGetLastError();
+}
+
+// prepare_A_local (field.h:1998-2075), operation for operation
+__global__ __launch_bounds__(256) void k_prepare_A_local(cplx* __restrict__ A, const cplx* __restrict__ xx, const cplx* __restrict__ xy,
+                                  int N, int coupling, int kernel, const double* __restrict__ q4,
+                                  const double* __restrict__ sb, const double* __restrict__ sa_k, Alpha al) {
+#pragma clang fp contract(off)
+    const size_t V = (size_t)N * N;
+    GRID_STRIDE(i, V) {
+        const int ix = (int)(i / N), iy = (int)(i % N);
+        int xxm = ix, yym = iy;
+        if (xxm >= N / 2) xxm -= N;
+        if (yym >= N / 2) yym -= N;
+        const double px = 2.0 * M_PI * xxm / (1.0 * N);
+        const double py = 2.0 * M_PI * yym / (1.0 * N);
+        cplx E[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) E[k] = mk(0.0, 0.0);
+        if (fabs(px * px + py * py) > 10e-9) {
+            double kxv, kyv;
+            if (kernel == 0) {
+                // LINEAR kernel: alpha at the continuum momentum (field.h:2032-2033)
+                double sc = 1.0;
+                if (coupling == 0)
+                    sc = sqrt(4.0 * M_PI / (al.beta0 * log(pow(pow(al.mu0_over_lambda * al.mu0_over_lambda,
+                                                                   1.0 / al.freeze_c) +
+                                                               pow(((px * px + py * py) * N * N) / al.lambda_L /
+                                                                   al.lambda_L, 1.0 / al.freeze_c),
+                                                               al.freeze_c))));
+                kxv = -2.0 * M_PI * sc * px / (px * px + py * py);
+                kyv = -2.0 * M_PI * sc * py / (px * px + py * py);
+            } else {
+                const double ph = q4[ix] + q4[iy];
+                const double sc = coupling == 0 ? sa_k[i] : 1.0;
+                kxv = -2.0 * M_PI * sc * sb[ix] / ph;
+                kyv = -2.0 * M_PI * sc * sb[iy] / ph;
+            }
+            const cplx KX = mk(0.0, kxv), KY = mk(0.0, kyv);
+#pragma unroll
+            for (int k = 0; k < 9; k++) E[k] = xadd(xmul(KX, xx[k * V + i]), xmul(KY, xy[k * V + i]));
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) A[k * V + i] = E[k];
+    }
+}
+
+hipError_t prepare_A_local(jw_ctx* c, cplx* A, const cplx* xix, const cplx* xiy, int coupling, int kernel,
+                           const double* sa_k, const Alpha& al) {
+    KTimer t(c, K_DROPIN);
+    hipLaunchKernelGGL(k_prepare_A_local, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, A, xix, xiy, c->N,
+                       coupling, kernel, c->q4, c->sb, sa_k, al);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ m3 load9(const cplx* __restrict__ f, size_t V, size_t i) {
+    m3 a;
+#pragma unroll
+    for (int k = 0; k < 9; k++) a.m[k] = f[k * V + i];
+    return a;
+}
+__device__ __forceinline__ void store9(cplx* __restrict__ f, size_t V, size_t i, const m3& a) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) f[k * V + i] = a.m[k];
+}
+
+// uxiulocal (field.h:1892-1933): (U xi) U^dagger, operation for operation
+__global__ __launch_bounds__(256) void k_uxiulocal(cplx* __restrict__ vx, cplx* __restrict__ vy, const cplx* __restrict__ U,
+                            const cplx* __restrict__ xx, const cplx* __restrict__ xy, size_t V) {
+    GRID_STRIDE(i, V) {
+        const m3 u = load9(U, V, i);
+        m3 d;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) d.m[3 * r + cc] = conjg(u.m[3 * cc + r]);
+        store9(vx, V, i, xmmul(xmmul(u, load9(xx, V, i)), d));
+        store9(vy, V, i, xmmul(xmmul(u, load9(xy, V, i)), d));
+    }
+}
+
+hipError_t uxiulocal(jw_ctx* c, cplx* Vx, cplx* Vy, const cplx* U, const cplx* xix, const cplx* xiy) {
+    KTimer t(c, K_DROPIN);
+    hipLaunchKernelGGL(k_uxiulocal, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, Vx, Vy, U, xix, xiy, c->V);
+    return hipGetLastError();
+}
+
+// update_uf (field.h:1966-1996): U <- exp(i sqrt(ds) B) U exp(-i sqrt(ds) A), both
+// exponents read through their upper triangles
+__global__ __launch_bounds__(256) void k_update_uf(cplx* __restrict__ U, const cplx* __restrict__ B, const cplx* __restrict__ A, size_t V,
+                            double ss) {
+    GRID_STRIDE(i, V) {
+        double hb[8], ha[8];
+        gen_from_upper(load9(B, V, i), hb);
+        gen_from_upper(load9(A, V, i), ha);
+        const m3 eb = exp_i_gen(hb, ss);
+        const m3 ea = exp_i_gen(ha, -ss);
+        store9(U, V, i, mmul(mmul(eb, load9(U, V, i)), ea));
+    }
+}
+
+hipError_t update_uf(jw_ctx* c, cplx* U, const cplx* B, const cplx* A, double step) {
+    KTimer t(c, K_DROPIN);
+    hipLaunchKernelGGL(k_update_uf, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, U, B, A, c->V, sqrt(step));
+    return hipGetLastError();
+}
+
+// setMVModel (field.h:683-757): n = z*sigma + 0, lambda-basis map
+__global__ __launch_bounds__(256) void k_set_mv_model(cplx* __restrict__ f, size_t V, NoiseSrc ns, double sigma) {
+#pragma clang fp contract(off)
+    GRID_STRIDE(i, V) {
+        double z[8], n[8];
+        normals8(ns, i, z);
+#pragma unroll
+        for (int k = 0; k < 8; k++) n[k] = z[k] * sigma + 0.0;
+        cplx a[9];
+        lambda_entries(n, a);
+#pragma unroll
+        for (int k = 0; k < 9; k++) f[k * V + i] = a[k];
+    }
+}
+
+hipError_t set_mv_model(jw_ctx* c, cplx* f, const NoiseSrc& ns, double sigma) {
+    KTimer t(c, K_DROPIN);
+    hipLaunchKernelGGL(k_set_mv_model, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, f, c->V, ns, sigma);
+    return hipGetLastError();
+}
+
+// solvePoisson (field.h:867-877): f(k) *= -g/(-phat2(k) + mass^2)
+__global__ __launch_bounds__(256) void k_solve_poisson(cplx* __restrict__ f, int t, int N, const double* __restrict__ q4, double mass,
+                                double g) {
+#pragma clang fp contract(off)
+    const size_t V = (size_t)N * N;
+    GRID_STRIDE(i, V) {
+        const double ph = q4[i / N] + q4[i % N];
+        const double fac = -1.0 * g / (-ph + mass * mass);
+        for (int k = 0; k < t; k++) {
+            const cplx v = f[k * V + i];
+            f[k * V + i] = xmul(v, mk(fac, 0.0));
+        }
+    }
+}
+
+hipError_t solve_poisson(jw_ctx* c, cplx* f, int t, double mass, double g) {
+    KTimer t_(c, K_DROPIN);
+    hipLaunchKernelGGL(k_solve_poisson, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, f, t, c->N, c->q4,
+                       mass, g);
+    return hipGetLastError();
+}
+
+// operator*= (field.h:337-365): U <- U exp(i f)
+__global__ __launch_bounds__(256) void k_mul_exp(cplx* __restrict__ U, const cplx* __restrict__ f, size_t V) {
+    GRID_STRIDE(i, V) {
+        double h[8];
+        gen_from_upper(load9(f, V, i), h);
+        store9(U, V, i, mmul(load9(U, V, i), exp_i_gen(h, 1.0)));
+    }
+}
+
+hipError_t mul_exp(jw_ctx* c, cplx* U, const cplx* f) {
+    KTimer t(c, K_DROPIN);
+    hipLaunchKernelGGL(k_mul_exp, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, U, f, c->V);
+    return hipGetLastError();
+}
+
+// lfield::trace (field.h:1375-1432): tr[(U/V)(U/V)^dagger], operation for operation
+__device__ __forceinline__ double trace_site(const cplx* __restrict__ U, size_t V, size_t i, double vol) {
+#pragma clang fp contract(off)
+    double diag[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const cplx u = U[(3 * r + k) * V + i];
+            const double ax = u.x / vol, ay = u.y / vol;
+            s = s + (ax * ax - ay * (-ay));
+        }
+        diag[r] = s;
+    }
+    return (diag[0] + diag[1]) + diag[2];
+}
+
+__global__ __launch_bounds__(256) void k_trace(const cplx* __restrict__ U, cplx* __restrict__ corr, size_t V, double vol) {
+    GRID_STRIDE(i, V) corr[i] = mk(trace_site(U, V, i, vol), 0.0);
+}
+
+hipError_t trace(jw_ctx* c, const cplx* U, cplx* corr) {
+    KTimer t(c, K_OBS);
+    hipLaunchKernelGGL(k_trace, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, U, corr, c->V,
+                       1.0 * c->N * c->N);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_corr_from_Uk(const cplx* __restrict__ U, double* __restrict__ corr, size_t V, double vol) {
+    GRID_STRIDE(i, V) corr[i] = trace_site(U, V, i, vol);
+}
+
+hipError_t corr_from_Uk(jw_ctx* c, const cplx* Uk, double* corr) {
+    KTimer t(c, K_OBS);
+    hipLaunchKernelGGL(k_corr_from_Uk, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, Uk, corr, c->V,
+                       1.0 * c->N * c->N);
+    return hipGetLastError();
+}
+
+// average_and_symmetrize (field.h:1460-1492): transpose average, then the
+// (+-kx, +-ky) reflection average, operation for operation (out of place)
+__global__ __launch_bounds__(256) void k_symmetrize(const cplx* __restrict__ in, cplx* __restrict__ out, int N) {
+#pragma clang fp contract(off)
+    const size_t V = (size_t)N * N;
+    GRID_STRIDE(idx, V) {
+        const int i = (int)(idx / N), j = (int)(idx % N);
+        const int a = abs(i - N) % N, b = abs(j - N) % N;
+        auto tmp = [&](int x, int y) {
+            const cplx p = in[(size_t)x * N + y], q = in[(size_t)y * N + x];
+            return mk(0.5 * (p.x + q.x), 0.5 * (p.y + q.y));
+        };
+        const cplx t1 = tmp(i, j), t2 = tmp(a, j), t3 = tmp(i, b), t4 = tmp(a, b);
+        const cplx s = xadd(xadd(xadd(t1, t2), t3), t4);
+        out[idx] = mk(0.25 * s.x, 0.25 * s.y);
+    }
+}
+
+hipError_t average_and_symmetrize(jw_ctx* c, const cplx* in, cplx* out) {
+    KTimer t(c, K_OBS);
+    hipLaunchKernelGGL(k_symmetrize, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, in, out, c->N);
+    return hipGetLastError();
+}
+
+// reduce (field.h:1521-1536): sum += c, err += c^2
+__global__ __launch_bounds__(256) void k_reduce(const cplx* __restrict__ corr, cplx* __restrict__ sum, cplx* __restrict__ err, size_t V) {
+#pragma clang fp contract(off)
+    GRID_STRIDE(i, V) {
+        const cplx v = corr[i];
+        sum[i] = mk(sum[i].x + v.x, sum[i].y + v.y);
+        const cplx sq = (v.y == 0.0 && v.x > 0.0) ? mk(v.x * v.x, 0.0) : xmul(v, v);
+        err[i] = mk(err[i].x + sq.x, err[i].y + sq.y);
+    }
+}
+
+hipError_t reduce(jw_ctx* c, const cplx* corr, cplx* sum, cplx* err) {
+    KTimer t(c, K_OBS);
+    hipLaunchKernelGGL(k_reduce, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, corr, sum, err, c->V);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_accumulate_real(const double* __restrict__ corr, double* __restrict__ sum, double* __restrict__ err,
+                                  size_t V) {
+#pragma clang fp contract(off)
+    GRID_STRIDE(i, V) {
+        const double v = corr[i];
+        sum[i] += v;
+        err[i] += v * v;
+    }
+}
+
+hipError_t accumulate_real(jw_ctx* c, const double* corr, double* sum, double* err) {
+    KTimer t(c, K_OBS);
+    hipLaunchKernelGGL(k_accumulate_real, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, corr, sum, err, c->V);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_scale(cplx* __restrict__ p, size_t n, double s) {
+    GRID_STRIDE(i, n) p[i] = cscale(p[i], s);
+}
+
+hipError_t scale_planes(jw_ctx* c, cplx* p, size_t n, double s) {
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, p, n, s);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// generator-basis engine (packed planes)
+//   Xi planes 0..3: (n0,n1),(n2,n3),(n4,n5),(n6,n7) of xi_x; planes 4..7: same for xi_y
+//   Vv planes: the same layout for V_x, V_y;  ZA, ZB: 4 planes (A, B)
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_gen_noise_packed(cplx* __restrict__ Xi, size_t V, NoiseSrc ns, const double* __restrict__ sa_pos) {
+    GRID_STRIDE(i, V) {
+        double z[16];
+        normals16(ns, i, z);
+        const double s = sa_pos ? sa_pos[i] : 1.0;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            Xi[p * V + i] = mk(z[4 * p] * s, z[4 * p + 2] * s);           // n_{2p}, n_{2p+1}
+            Xi[(4 + p) * V + i] = mk(z[4 * p + 1] * s, z[4 * p + 3] * s); // m_{2p}, m_{2p+1}
+        }
+    }
+}
+
+hipError_t gen_noise_packed(jw_ctx* c, cplx* Xi, const NoiseSrc& ns, const double* sa_pos) {
+    KTimer t(c, K_GEN_NOISE);
+    hipLaunchKernelGGL(k_gen_noise_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, Xi, c->V, ns, sa_pos);
+    return hipGetLastError();
+}
+
+// Z[p] = i (kx(k) X[p] + ky(k) Y[p]),  kx = -2 pi c pbarX / phat2, zero at k = 0;
+// scale_inputs: X, Y *= sa(k) first (and written back) for per-mode correlated noise.
+__global__ __launch_bounds__(256) void k_kernel_mult_packed(cplx* __restrict__ Z, cplx* __restrict__ X, cplx* __restrict__ Y, int N,
+                                     int coupling, const double* __restrict__ q4, const double* __restrict__ sb,
+                                     const double* __restrict__ sa, int scale_inputs) {
+    const size_t V = (size_t)N * N;
+    GRID_STRIDE(i, V) {
+        const int ix = (int)(i / N), iy = (int)(i % N);
+        const double ph = q4[ix] + q4[iy];
+        const double sc = (coupling == 0) ? sa[i] : 1.0;
+        const double kxv = (i == 0) ? 0.0 : -2.0 * M_PI * sc * sb[ix] / ph;
+        const double kyv = (i == 0) ? 0.0 : -2.0 * M_PI * sc * sb[iy] / ph;
+        const double si = scale_inputs ? sa[i] : 1.0;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            cplx x = X[p * V + i], y = Y[p * V + i];
+            if (scale_inputs) {
+                x = cscale(x, si);
+                y = cscale(y, si);
+                X[p * V + i] = x;
+                Y[p * V + i] = y;
+            }
+            const cplx s = mk(kxv * x.x + kyv * y.x, kxv * x.y + kyv * y.y);
+            Z[p * V + i] = cmuli(s);
+        }
+    }
+}
+
+hipError_t kernel_mult_packed(jw_ctx* c, cplx* Z, const cplx* X, const cplx* Y, int coupling, const double* sa_k,
+                              int scale_inputs) {
+    KTimer t(c, K_KMULT);
+    hipLaunchKernelGGL(k_kernel_mult_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, Z,
+                       const_cast<cplx*>(X), const_cast<cplx*>(Y), c->N, coupling, c->q4, c->sb, sa_k, scale_inputs);
+    return hipGetLastError();
+}
+
+// V_i = U xi_i U^dagger in generator coefficients (upper triangle + real diagonal)
+__device__ __forceinline__ void uxiu_gen(const m3& u, const double h[8], double out[8]) {
+    const m3 X = herm_from_gen(h, 1.0);
+    const m3 W = mmul(u, X);
+    m3 Vm;
+    // only the entries gen_from_upper reads: diagonal and upper triangle
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int cc = r; cc < 3; cc++) {
+            cplx s = cmulc(W.m[3 * r], u.m[3 * cc]);
+            s = cadd(s, cmulc(W.m[3 * r + 1], u.m[3 * cc + 1]));
+            s = cadd(s, cmulc(W.m[3 * r + 2], u.m[3 * cc + 2]));
+            Vm.m[3 * r + cc] = s;
+        }
+    gen_from_upper(Vm, out);
+}
+
+__global__ __launch_bounds__(256) void k_uxiu_packed(cplx* __restrict__ Vv, const cplx* __restrict__ U, const cplx* __restrict__ Xi_pos,
+                              size_t V, NoiseSrc ns, const double* __restrict__ sa_pos) {
+    GRID_STRIDE(i, V) {
+        double hx[8], hy[8];
+        if (Xi_pos) {
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const cplx a = Xi_pos[p * V + i], b = Xi_pos[(4 + p) * V + i];
+                hx[2 * p] = a.x; hx[2 * p + 1] = a.y;
+                hy[2 * p] = b.x; hy[2 * p + 1] = b.y;
+            }
+        } else {
+            double z[16];
+            normals16(ns, i, z);
+            const double s = sa_pos ? sa_pos[i] : 1.0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) { hx[k] = z[2 * k] * s; hy[k] = z[2 * k + 1] * s; }
+        }
+        const m3 u = load9(U, V, i);
+        double vx[8], vy[8];
+        uxiu_gen(u, hx, vx);
+        uxiu_gen(u, hy, vy);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            Vv[p * V + i] = mk(vx[2 * p], vx[2 * p + 1]);
+            Vv[(4 + p) * V + i] = mk(vy[2 * p], vy[2 * p + 1]);
+        }
+    }
+}
+
+hipError_t uxiu_packed(jw_ctx* c, cplx* Vv, const cplx* U, const cplx* Xi_pos, const NoiseSrc& ns,
+                       const double* sa_pos) {
+    KTimer t(c, K_UXIU);
+    hipLaunchKernelGGL(k_uxiu_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, Vv, U, Xi_pos, c->V, ns,
+                       sa_pos);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_update_packed(cplx* __restrict__ U, const cplx* __restrict__ ZA, const cplx* __restrict__ ZB,
+                                size_t V, double ss) {
+    GRID_STRIDE(i, V) {
+        double ha[8], hb[8];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const cplx a = ZA[p * V + i], b = ZB[p * V + i];
+            ha[2 * p] = a.x; ha[2 * p + 1] = a.y;
+            hb[2 * p] = b.x; hb[2 * p + 1] = b.y;
+        }
+        const m3 eb = exp_i_gen(hb, ss);
+        const m3 ea = exp_i_gen(ha, -ss);
+        store9(U, V, i, mmul(mmul(eb, load9(U, V, i)), ea));
+    }
+}
+
+hipError_t update_packed(jw_ctx* c, cplx* U, const cplx* ZA, const cplx* ZB, double sqrt_step) {
+    KTimer t(c, K_UPDATE);
+    hipLaunchKernelGGL(k_update_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, U, ZA, ZB, c->V,
+                       sqrt_step);
+    return hipGetLastError();
+}
+
+// MV sheets in the generator basis
+__global__ __launch_bounds__(256) void k_mv_noise_packed(cplx* __restrict__ R, size_t V, NoiseSrc ns, double sigma) {
+    GRID_STRIDE(i, V) {
+        double z[8];
+        normals8(ns, i, z);
+#pragma unroll
+        for (int p = 0; p < 4; p++) R[p * V + i] = mk(z[2 * p] * sigma + 0.0, z[2 * p + 1] * sigma + 0.0);
+    }
+}
+
+hipError_t mv_noise_packed(jw_ctx* c, cplx* R, const NoiseSrc& ns, double sigma) {
+    KTimer t(c, K_MV);
+    hipLaunchKernelGGL(k_mv_noise_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, R, c->V, ns, sigma);
+    return hipGetLastError();
+}
+
+hipError_t poisson_packed(jw_ctx* c, cplx* R, double mass, double g) { return solve_poisson(c, R, 4, mass, g); }
+
+__global__ __launch_bounds__(256) void k_mul_exp_packed(cplx* __restrict__ U, const cplx* __restrict__ R, size_t V) {
+    GRID_STRIDE(i, V) {
+        double h[8];
+#pragma unroll
+        for (int p = 0; p < 4; p++) { const cplx a = R[p * V + i]; h[2 * p] = a.x; h[2 * p + 1] = a.y; }
+        store9(U, V, i, mmul(load9(U, V, i), exp_i_gen(h, 1.0)));
+    }
+}
+
+hipError_t mul_exp_packed(jw_ctx* c, cplx* U, const cplx* R) {
+    KTimer t(c, K_MV);
+    hipLaunchKernelGGL(k_mul_exp_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, U, R, c->V);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// unit-test hook: exp(i s H(h)) for a batch of generator vectors
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_exp_batch(const double* __restrict__ h8, double s, double* __restrict__ out, int n) {
+    GRID_STRIDE(i, (size_t)n) {
+        double h[8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) h[a] = h8[8 * i + a];
+        const m3 e = exp_i_gen(h, s);
+#pragma unroll
+        for (int k = 0; k < 9; k++) { out[18 * i + 2 * k] = e.m[k].x; out[18 * i + 2 * k + 1] = e.m[k].y; }
+    }
+}
+
+hipError_t exp_batch(jw_ctx* c, const double* h8, double s, double* out18, int n) {
+    hipLaunchKernelGGL(k_exp_batch, dim3(grid_for((size_t)n, 256)), dim3(256), 0, c->stream, h8, s, out18, n);
+    return hipGetLastError();
+}
+
+}  // namespace jw

@@ -1,0 +1,256 @@
+"""Python binding of the MI355X JIMWLK library (libjimwlk_hip.so, C ABI in
+include/jimwlk.h), for tests and benchmarks.
+
+Host fields are numpy complex128 arrays of shape (N*N, t) in the reference's
+AoS layout (site x*N + y, entry k = 3*row + col; field.h:199-209).  There is
+no CPU fallback: loading fails when the HIP library is missing.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjimwlk_hip.so")
+
+SQRT_COUPLING_CONSTANT, NOISE_COUPLING_CONSTANT, HATTA_COUPLING_CONSTANT = 0, 1, 2
+FIXED_COUPLING, CORRELATED_NOISE_COUPLING = 3, 4
+LINEAR_KERNEL, SIN_KERNEL = 0, 1
+TAG_LANGEVIN, TAG_MV = 0x4C414E47, 0x4D565348
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class JwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"jimwlk error {code}: {msg}")
+        self.code = code
+
+
+class Alpha(ctypes.Structure):
+    _fields_ = [("lambda_L", ctypes.c_double), ("mu0_over_lambda", ctypes.c_double),
+                ("freeze_c", ctypes.c_double), ("beta0", ctypes.c_double)]
+
+
+def alpha(lambda_L=6.0, mu0_over_lambda=2.5, freeze_c=0.2, beta0=9.0):
+    return Alpha(lambda_L, mu0_over_lambda, freeze_c, beta0)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "lib"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.jw_last_error.restype = ctypes.c_char_p
+        L.jw_version.restype = ctypes.c_char_p
+        L.jw_kernel_name.restype = ctypes.c_char_p
+        L.jw_destroy.restype = None
+        L.jw_field_destroy.restype = None
+        L.jw_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int]
+        L.jw_field_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        for name in ("jw_destroy", "jw_sync", "jw_lattice_size"):
+            getattr(L, name).argtypes = [ctypes.c_void_p]
+        L.jw_field_destroy.argtypes = [ctypes.c_void_p]
+        L.jw_field_upload.argtypes = [ctypes.c_void_p, _dp]
+        L.jw_field_download.argtypes = [ctypes.c_void_p, _dp]
+        L.jw_noise_inject.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        L.jw_noise_philox.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.jw_evolve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                ctypes.POINTER(Alpha), ctypes.c_int]
+        L.jw_mv_init.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, _dp]
+        L.jw_update_uf.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double]
+        L.jw_set_mv_model.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.jw_solve_poisson.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.jw_prepare_A_local.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Alpha)]
+        L.jw_generate_gaussian_with_noise_coupling_constant.argtypes = [ctypes.c_void_p] * 3 + [ctypes.POINTER(Alpha)]
+        L.jw_measure_corr.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp]
+        L.jw_dipole_from_corr.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.jw_radial_bins.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]
+        L.jw_saturation_scale.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.jw_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]
+        L.jw_su3_exp_gen.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, ctypes.c_int]
+        L.jw_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise JwError(rc, lib().jw_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Field:
+    def __init__(self, ctx, t=9):
+        self.ctx, self.t = ctx, t
+        h = ctypes.c_void_p()
+        _check(lib().jw_field_create(ctx.h, t, ctypes.byref(h)))
+        self.h = h
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128).reshape(self.ctx.V, self.t)
+        _check(lib().jw_field_upload(self.h, _p(a)))
+        return self
+
+    def download(self):
+        a = np.zeros((self.ctx.V, self.t), np.complex128)
+        _check(lib().jw_field_download(self.h, _p(a)))
+        return a
+
+    def close(self):
+        if self.h:
+            lib().jw_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One lattice (N x N) on one GPU (jw_create)."""
+
+    def __init__(self, N, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().jw_create(ctypes.byref(h), N, device))
+        self.h, self.N, self.V = h, N, N * N
+
+    def close(self):
+        if self.h:
+            lib().jw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field(self, t=9, data=None):
+        f = Field(self, t)
+        if data is not None:
+            f.upload(data)
+        return f
+
+    def sync(self):
+        _check(lib().jw_sync(self.h))
+
+    # noise
+    def inject(self, z, per_site=16):
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        assert z.size == per_site * self.V
+        _check(lib().jw_noise_inject(self.h, _p(z), per_site))
+
+    def philox(self, seed, traj=0, step=0):
+        _check(lib().jw_noise_philox(self.h, seed, traj, step))
+
+    # reference per-function entry points
+    def execute2D(self, f, dir):
+        _check(lib().jw_execute2D(self.h, f.h, int(dir)))
+
+    def generate_gaussian(self, xx, xy):
+        _check(lib().jw_generate_gaussian(self.h, xx.h, xy.h))
+
+    def generate_gaussian_with_noise_coupling_constant(self, xx, xy, al=None):
+        _check(lib().jw_generate_gaussian_with_noise_coupling_constant(self.h, xx.h, xy.h,
+                                                                        ctypes.byref(al) if al else None))
+
+    def prepare_A_local(self, A, xx, xy, coupling, kernel, al=None):
+        _check(lib().jw_prepare_A_local(self.h, A.h, xx.h, xy.h, coupling, kernel, ctypes.byref(al) if al else None))
+
+    def uxiulocal(self, vx, vy, U, xx, xy):
+        _check(lib().jw_uxiulocal(self.h, vx.h, vy.h, U.h, xx.h, xy.h))
+
+    def update_uf(self, U, B, A, step):
+        _check(lib().jw_update_uf(self.h, U.h, B.h, A.h, step))
+
+    def set_mv_model(self, f, g, mu, Ny):
+        _check(lib().jw_set_mv_model(self.h, f.h, g, mu, Ny))
+
+    def solve_poisson(self, f, mass, g):
+        _check(lib().jw_solve_poisson(self.h, f.h, mass, g))
+
+    def mul_exp(self, U, f):
+        _check(lib().jw_mul_exp(self.h, U.h, f.h))
+
+    def set_unit(self, f):
+        _check(lib().jw_set_to_unit(f.h))
+
+    def trace(self, U, corr):
+        _check(lib().jw_trace(self.h, U.h, corr.h))
+
+    def average_and_symmetrize(self, corr):
+        _check(lib().jw_average_and_symmetrize(self.h, corr.h))
+
+    def reduce(self, corr, s, e):
+        _check(lib().jw_reduce(self.h, corr.h, s.h, e.h))
+
+    # engine
+    def evolve(self, U, coupling=FIXED_COUPLING, kernel=SIN_KERNEL, step=1e-4, al=None, nsteps=1):
+        _check(lib().jw_evolve(self.h, U.h, coupling, kernel, step, ctypes.byref(al) if al else None, nsteps))
+
+    def mv_init(self, U, Ny, g, mu, mass, z=None):
+        if z is not None:
+            z = np.ascontiguousarray(z, dtype=np.float64)
+            assert z.size == 8 * self.V * Ny
+        _check(lib().jw_mv_init(self.h, U.h, Ny, g, mu, mass, _p(z) if z is not None else None))
+
+    # observables
+    def measure_corr(self, U):
+        c = np.zeros(self.V)
+        _check(lib().jw_measure_corr(self.h, U.h, _p(c)))
+        return c
+
+    def dipole_from_corr(self, corr):
+        corr = np.ascontiguousarray(corr, dtype=np.float64)
+        S = np.zeros(self.V)
+        _check(lib().jw_dipole_from_corr(self.h, _p(corr), _p(S)))
+        return S
+
+    # instrumentation
+    def timing(self, enable=True):
+        _check(lib().jw_timing(self.h, 1 if enable else 0))
+
+    def kernel_times(self):
+        out = {}
+        for k in range(9):
+            ms, n = ctypes.c_double(), ctypes.c_long()
+            _check(lib().jw_kernel_time(self.h, k, ctypes.byref(ms), ctypes.byref(n)))
+            if n.value:
+                out[lib().jw_kernel_name(k).decode()] = (ms.value, n.value)
+        return out
+
+    def su3_exp_gen(self, h8, s):
+        h8 = np.ascontiguousarray(h8, dtype=np.float64).reshape(-1, 8)
+        out = np.zeros((h8.shape[0], 9), np.complex128)
+        _check(lib().jw_su3_exp_gen(self.h, _p(h8), s, _p(out), h8.shape[0]))
+        return out.reshape(-1, 3, 3)
+
+
+def radial_bins(S, N, nbins):
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    r, s, c = np.zeros(nbins), np.zeros(nbins), np.zeros(nbins)
+    _check(lib().jw_radial_bins(N, _p(S), nbins, _p(r), _p(s), _p(c)))
+    return r, s, c
+
+
+def saturation_scale(r, s, c):
+    q = ctypes.c_double()
+    r, s, c = (np.ascontiguousarray(a, dtype=np.float64) for a in (r, s, c))
+    _check(lib().jw_saturation_scale(len(r), _p(r), _p(s), _p(c), ctypes.byref(q)))
+    return q.value
