@@ -89,26 +89,34 @@ def main():
     ctx.evolve(U, coupling, jw.SIN_KERNEL, step, al, nsteps=min(a.steps, 10))
     kt = ctx.kernel_times()
     ctx.timing(False)
-    # dominant kernel class by total time per step
+    # Algorithmic HBM bytes per site per step of each kernel class of the fused
+    # step (complex planes of 16 B each way; U is 9 complex planes):
+    #   row_noise_fwd  W 8 planes                                  128
+    #   col_kmult      A: R 8 W 4 (+W 8 correlated), B: R 8 W 4    384 (512)
+    #   row_uxiu_fwd   R U 9 (+ xi~ 8 correlated), W 8             272 (400)
+    #   fft_rows       inverse rows of A,B (R 8 W 8) (+ xi~)       256 (512)
+    #   update         R U 9 + A 4 + B 4, W U 9                    416
+    kind = "correlated" if a.coupling == "correlated" else "white"
+    corr_ = kind == "correlated"
+    alg = {"row_noise_fwd": 128, "col_kmult": 512 if corr_ else 384, "row_uxiu_fwd": 400 if corr_ else 272,
+           "fft_rows": 512 if corr_ else 256, "update": 416}
     V = N * N
     nk = min(a.steps, 10)
-    tot = {k: ms * n / nk for k, (ms, n) in kt.items()}
-    dom = max(tot, key=tot.get)
-    dom_ms, dom_calls = kt[dom]
-    # algorithmic bytes per launch of the FFT passes: every plane element read
-    # and written once (16 B complex each way); planes per launch averaged
-    planes_per_launch = {"correlated": (8 + 4 + 8 + 8 + 4) / 5.0, "white": (8 + 4 + 8 + 4) / 4.0}
-    kind = "correlated" if a.coupling == "correlated" else "white"
-    if dom in ("fft_rows", "fft_cols"):
-        alg_bytes = 32.0 * V * planes_per_launch[kind]
-    else:
-        alg_bytes = None
+    per_step_ms = {k: ms * n / nk for k, (ms, n) in kt.items()}
+    dom = max(per_step_ms, key=per_step_ms.get)
+    classes = {}
+    for k, t in per_step_ms.items():
+        if k in alg:
+            classes[k] = {"ms_per_step": round(t, 5), "GBs": round(alg[k] * V / (t * 1e-3) / 1e9, 1),
+                          "launch_ms": round(kt[k][0], 5), "launches_per_step": kt[k][1] / nk}
     roof = None
-    if alg_bytes:
-        ach = alg_bytes / (dom_ms * 1e-3) / 1e9
+    if dom in alg:
+        dom_ms, dom_calls = kt[dom]
+        bytes_per_launch = alg[dom] * V * nk / dom_calls
+        ach = bytes_per_launch / (dom_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "avg_launch_ms": round(dom_ms, 5), "alg_bytes_per_launch": alg_bytes}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(dom_ms, 5),
+                "alg_bytes_per_launch": bytes_per_launch}
     step_bytes = REALS_PER_SITE[kind] * 8.0 * V
     step_frac = step_bytes * sps / (HBM_PEAK_GBS * 1e9)
     print(f"bench: kernel classes {json.dumps({k: [round(v[0], 4), v[1]] for k, v in kt.items()})}",
@@ -147,7 +155,7 @@ def main():
                       "form": "plain FP64", "parallelism": "single GPU"},
            "step_roofline": {"bytes_per_step": step_bytes, "achieved_GBs": round(step_bytes * sps / 1e9, 1),
                              "frac": round(step_frac, 4), "reals_per_site": REALS_PER_SITE[kind]},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "kernels": classes, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     ctx.close()
 

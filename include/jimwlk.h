@@ -163,6 +163,9 @@ int jw_radial_bins(int N, const double* S, int nbins, double* rbar, double* Sbin
 int jw_saturation_scale(int nbins, const double* rbar, const double* Sbin, const double* count, double* Qs);
 
 /* ---- instrumentation ------------------------------------------------------ */
+/* Select the fused row/column passes (default, 64 <= N <= 2048) or the unfused
+ * kernel sequence for jw_evolve; both compute the same step. */
+int jw_set_fused(jw_ctx* ctx, int enable);
 /* Enable per-kernel timing with HIP events on the context's stream. */
 int jw_timing(jw_ctx* ctx, int enable);
 /* Average duration (ms) and call count of kernel class `which` (see jw_kernel_name). */

@@ -429,10 +429,15 @@ int jw_evolve(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, con
     const int kc = coupling == JW_SQRT_COUPLING_CONSTANT ? 0 : 1;
     const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
     const double s = 1.0 / c->N, ss = sqrt(step);
+    const bool fused = jw::fused_supported(c->N) && !c->unfused;
     for (int it = 0; it < nsteps; it++) {
         bool ok;
         jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
         if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
+        if (fused) {
+            HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc == 0, corr, ss));
+            continue;
+        }
         HIP_CHECK(jw::gen_noise_packed(c, Xi, ns, sa_pos));
         HIP_CHECK(jw::fft2d(c, Xi, 8, 1, s));
         HIP_CHECK(jw::kernel_mult_packed(c, ZA, Xi, Xi + 4 * V, kc, sa_k, corr));
@@ -547,7 +552,14 @@ int jw_saturation_scale(int nbins, const double* rbar, const double* Sb, const d
 
 // ---------------------------------------------------------------- instrumentation
 static const char* kNames[] = {"fft_rows", "fft_cols", "gen_noise", "kernel_mult", "uxiu",
-                               "update", "mv", "observables", "reference_functions"};
+                               "update", "mv", "observables", "reference_functions",
+                               "row_noise_fwd", "col_kmult", "row_uxiu_fwd"};
+
+int jw_set_fused(jw_ctx* c, int enable) {
+    if (!c) return fail(JW_ERR_ARG, "null context");
+    c->unfused = enable ? 0 : 1;
+    return JW_OK;
+}
 
 int jw_timing(jw_ctx* c, int enable) {
     if (!c) return fail(JW_ERR_ARG, "null context");

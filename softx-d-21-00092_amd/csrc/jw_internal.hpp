@@ -55,6 +55,8 @@ struct jw_ctx {
     // workspace: generator-basis planes (packed) and scratch complex planes
     jw::cplx* ws;
     size_t ws_planes;
+    // 1: evolve through the unfused kernel sequence even where the fused passes exist
+    int unfused;
     // per-kernel timing (HIP events on ctx->stream)
     int timing;
     double kernel_ms[16];
@@ -95,6 +97,11 @@ hipError_t update_packed(jw_ctx* c, cplx* U, const cplx* ZA, const cplx* ZB, dou
 hipError_t mv_noise_packed(jw_ctx* c, cplx* R, const NoiseSrc& ns, double sigma);
 hipError_t poisson_packed(jw_ctx* c, cplx* R, double mass, double g);
 hipError_t mul_exp_packed(jw_ctx* c, cplx* U, const cplx* R);
+
+// fused generator-basis step (64 <= N <= 2048)
+bool fused_supported(int N);
+hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
+                      const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr, double sqrt_step);
 
 // observables
 hipError_t corr_from_Uk(jw_ctx* c, const cplx* Uk, double* corr);

@@ -11,75 +11,11 @@
 //  * the generator-basis engine: every su(3) field (xi_x, xi_y, A, V_x, V_y, B)
 //    is carried as 8 real generator planes packed two per complex plane, which
 //    turns the reference's 72 complex plane FFTs per step into 24.
-#include "jw_device.hpp"
-#include "jw_internal.hpp"
+#include "jw_common.hpp"
 
 #include <math.h>
 
 namespace jw {
-
-// ---------------------------------------------------------------------------
-// timing of kernel classes with HIP events on the context stream
-// ---------------------------------------------------------------------------
-enum KClass { K_FFT_ROWS = 0, K_FFT_COLS, K_GEN_NOISE, K_KMULT, K_UXIU, K_UPDATE, K_MV, K_OBS, K_DROPIN,
-              K_NCLASS };
-
-struct KTimer {
-    jw_ctx* c;
-    int cls;
-    hipEvent_t a, b;
-    KTimer(jw_ctx* c_, int cls_) : c(c_), cls(cls_) {
-        if (c->timing) {
-            hipEventCreate(&a);
-            hipEventCreate(&b);
-            hipEventRecord(a, c->stream);
-        }
-    }
-    ~KTimer() {
-        if (c->timing) {
-            hipEventRecord(b, c->stream);
-            hipEventSynchronize(b);
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, a, b);
-            c->kernel_ms[cls] += ms;
-            c->kernel_calls[cls] += 1;
-            hipEventDestroy(a);
-            hipEventDestroy(b);
-        }
-    }
-};
-
-static inline unsigned grid_for(size_t n, unsigned block) {
-    size_t g = (n + block - 1) / block;
-    if (g > 65535u * 16u) g = 65535u * 16u;
-    return (unsigned)(g ? g : 1);
-}
-
-#define GRID_STRIDE(i, n) for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); \
-                               i += (size_t)gridDim.x * blockDim.x)
-
-// non-contracting complex helpers for the bit-matching kernels
-__device__ __forceinline__ cplx xmul(cplx a, cplx b) {
-#pragma clang fp contract(off)
-    return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ cplx xadd(cplx a, cplx b) {
-#pragma clang fp contract(off)
-    return mk(a.x + b.x, a.y + b.y);
-}
-__device__ __forceinline__ m3 xmmul(const m3& a, const m3& b) {
-    m3 c;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            cplx s = mk(0.0, 0.0);
-#pragma unroll
-            for (int k = 0; k < 3; k++) s = xadd(s, xmul(a.m[3 * i + k], b.m[3 * k + j]));
-            c.m[3 * i + j] = s;
-        }
-    return c;
-}
 
 // ===========================================================================
 // FFT passes
@@ -211,50 +147,6 @@ hipError_t build_sqrt_alpha(jw_ctx* c, int variant, const Alpha& a) {
 }
 
 // ===========================================================================
-// noise
-// ===========================================================================
-__device__ __forceinline__ void normals16(const NoiseSrc& ns, size_t site, double z[16]) {
-    if (ns.inj) {
-        const double2* p = reinterpret_cast<const double2*>(ns.inj + 16 * site);
-#pragma unroll
-        for (int q = 0; q < 8; q++) { const double2 v = p[q]; z[2 * q] = v.x; z[2 * q + 1] = v.y; }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-            box_muller_pair((uint32_t)site, ns.step, ns.tag, (uint32_t)q, ns.seed, ns.traj, z[2 * q], z[2 * q + 1]);
-    }
-}
-
-__device__ __forceinline__ void normals8(const NoiseSrc& ns, size_t site, double z[8]) {
-    if (ns.inj) {
-        const double2* p = reinterpret_cast<const double2*>(ns.inj + 8 * site);
-#pragma unroll
-        for (int q = 0; q < 4; q++) { const double2 v = p[q]; z[2 * q] = v.x; z[2 * q + 1] = v.y; }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 4; q++)
-            box_muller_pair((uint32_t)site, ns.step, ns.tag, (uint32_t)q, ns.seed, ns.traj, z[2 * q], z[2 * q + 1]);
-    }
-}
-
-// the reference's lambda-basis site map (field.h:2169-2231; setMVModel field.h:714-745),
-// operation for operation
-__device__ __forceinline__ void lambda_entries(const double n[8], cplx u[9]) {
-#pragma clang fp contract(off)
-    const double s3 = sqrt(3.0);
-    const double d = n[7] / s3;
-    u[0] = mk(n[2] + d, 0.0);
-    u[1] = mk(n[0], n[1]);
-    u[2] = mk(n[3], n[4]);
-    u[3] = mk(n[0], -n[1]);
-    u[4] = mk(-n[2] + d, 0.0);
-    u[5] = mk(n[5], n[6]);
-    u[6] = mk(n[3], -n[4]);
-    u[7] = mk(n[5], -n[6]);
-    u[8] = mk(-2.0 * n[7] / s3, 0.0);
-}
-
-// ===========================================================================
 // reference per-function kernels (9-component fields)
 // ===========================================================================
 __global__ __launch_bounds__(256) void k_set_unit(cplx* __restrict__ U, size_t V) {
@@ -348,17 +240,6 @@ hipError_t prepare_A_local(jw_ctx* c, cplx* A, const cplx* xix, const cplx* xiy,
     hipLaunchKernelGGL(k_prepare_A_local, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, A, xix, xiy, c->N,
                        coupling, kernel, c->q4, c->sb, sa_k, al);
     return hipGetLastError();
-}
-
-__device__ __forceinline__ m3 load9(const cplx* __restrict__ f, size_t V, size_t i) {
-    m3 a;
-#pragma unroll
-    for (int k = 0; k < 9; k++) a.m[k] = f[k * V + i];
-    return a;
-}
-__device__ __forceinline__ void store9(cplx* __restrict__ f, size_t V, size_t i, const m3& a) {
-#pragma unroll
-    for (int k = 0; k < 9; k++) f[k * V + i] = a.m[k];
 }
 
 // uxiulocal (field.h:1892-1933): (U xi) U^dagger, operation for operation
@@ -624,24 +505,6 @@ hipError_t kernel_mult_packed(jw_ctx* c, cplx* Z, const cplx* X, const cplx* Y, 
     hipLaunchKernelGGL(k_kernel_mult_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, Z,
                        const_cast<cplx*>(X), const_cast<cplx*>(Y), c->N, coupling, c->q4, c->sb, sa_k, scale_inputs);
     return hipGetLastError();
-}
-
-// V_i = U xi_i U^dagger in generator coefficients (upper triangle + real diagonal)
-__device__ __forceinline__ void uxiu_gen(const m3& u, const double h[8], double out[8]) {
-    const m3 X = herm_from_gen(h, 1.0);
-    const m3 W = mmul(u, X);
-    m3 Vm;
-    // only the entries gen_from_upper reads: diagonal and upper triangle
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-#pragma unroll
-        for (int cc = r; cc < 3; cc++) {
-            cplx s = cmulc(W.m[3 * r], u.m[3 * cc]);
-            s = cadd(s, cmulc(W.m[3 * r + 1], u.m[3 * cc + 1]));
-            s = cadd(s, cmulc(W.m[3 * r + 2], u.m[3 * cc + 2]));
-            Vm.m[3 * r + cc] = s;
-        }
-    gen_from_upper(Vm, out);
 }
 
 __global__ __launch_bounds__(256) void k_uxiu_packed(cplx* __restrict__ Vv, const cplx* __restrict__ U, const cplx* __restrict__ Xi_pos,

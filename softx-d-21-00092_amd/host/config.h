@@ -1,0 +1,119 @@
+// config.h -- run parameters and their file parser, the surface of the
+// reference's config.h:28-139 with the lattice size made a run-time value.
+//
+// The file format is the reference's: eleven "key = value" lines in a fixed
+// order (config.h:76-136; example file `config`).  As in the reference, mu,
+// mass and step are read through a float (config.h:79-93), so 1e-4 becomes
+// 9.99999974737875e-05.  Optional lines after the eleven select what the
+// reference fixes at compile time or does not have:
+//   N = 2048                lattice extent (reference: #define Nx/Ny 64)
+//   seed = 1                Philox key (the reference seeds from the clock)
+//   lambda_L = 6            Lambda*N of the running coupling (field.h:2273)
+//   mu0_over_lambda = 2.5   mu0/Lambda (field.h:2257)
+//   engine = fused          fused | reference (the per-function calls)
+#ifndef JW_HOST_CONFIG_H
+#define JW_HOST_CONFIG_H
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+
+enum Evolution { POSITION_EVOLUTION, MOMENTUM_EVOLUTION };
+
+// The reference's three choices (config.h:40) and two additions: FIXED_COUPLING
+// (kernel factor 1, white noise) and CORRELATED_NOISE_COUPLING (per-mode
+// sqrt(alpha(k)) on the noise spectrum).
+enum Coupling {
+    SQRT_COUPLING_CONSTANT,
+    NOISE_COUPLING_CONSTANT,
+    HATTA_COUPLING_CONSTANT,
+    FIXED_COUPLING,
+    CORRELATED_NOISE_COUPLING
+};
+
+enum Kernel { LINEAR_KERNEL, SIN_KERNEL };
+
+class config {
+   public:
+    int Nx = 64, Ny = 64;
+    int Nxl = 0, Nyl = 0;
+    int Nxl_buf = 0, Nyl_buf = 0;
+    int proc_x = 1, proc_y = 1;
+
+    unsigned seed = 1;
+    int stat = 4;
+    double mu = 30.72;
+    double mass = 0.0001;
+    int elementaryWilsonLines = 50;
+    std::string file_name = "output";
+    int langevin_steps = 100;
+    int measurements = 100;
+    double step = 0.0004;
+
+    Evolution EvolutionChoice = MOMENTUM_EVOLUTION;
+    Coupling CouplingChoice = NOISE_COUPLING_CONSTANT;
+    Kernel KernelChoice = SIN_KERNEL;
+
+    double lambda_L = 6.0;
+    double mu0_over_lambda = 2.5;
+    bool fused = true;
+
+    // Returns 1 on success, 0 when the file cannot be read or a required key is
+    // missing (the reference returns nothing and reads garbage).
+    int read_config_from_file(const std::string& path) {
+        FILE* f = fopen(path.c_str(), "r");
+        if (!f) {
+            fprintf(stderr, "config: cannot open %s\n", path.c_str());
+            return 0;
+        }
+        float tmp;
+        char buf[256];
+        int ok = 1;
+        ok &= fscanf(f, " stat = %i", &stat) == 1;
+        ok &= fscanf(f, " mu = %f", &tmp) == 1;
+        mu = tmp;
+        ok &= fscanf(f, " mass = %f", &tmp) == 1;
+        mass = tmp;
+        ok &= fscanf(f, " elementaryWilsonLines = %i", &elementaryWilsonLines) == 1;
+        ok &= fscanf(f, " file_name = %255s", buf) == 1;
+        file_name = buf;
+        ok &= fscanf(f, " langevin_steps = %i", &langevin_steps) == 1;
+        ok &= fscanf(f, " measurements = %i", &measurements) == 1;
+        ok &= fscanf(f, " step = %f", &tmp) == 1;
+        step = tmp;
+        char evolution[100], coupling[100], kernel[100];
+        ok &= fscanf(f, " EvolutionChoice = %99s", evolution) == 1;
+        ok &= fscanf(f, " CouplingChoice = %99s", coupling) == 1;
+        ok &= fscanf(f, " KernelChoice = %99s", kernel) == 1;
+        if (!ok) {
+            fprintf(stderr, "config: %s does not hold the eleven reference keys in order\n", path.c_str());
+            fclose(f);
+            return 0;
+        }
+        if (!strcmp(evolution, "MOMENTUM_EVOLUTION")) EvolutionChoice = MOMENTUM_EVOLUTION;
+        if (!strcmp(evolution, "POSITION_EVOLUTION")) EvolutionChoice = POSITION_EVOLUTION;
+        if (!strcmp(coupling, "SQRT_COUPLING_CONSTANT")) CouplingChoice = SQRT_COUPLING_CONSTANT;
+        if (!strcmp(coupling, "NOISE_COUPLING_CONSTANT")) CouplingChoice = NOISE_COUPLING_CONSTANT;
+        if (!strcmp(coupling, "HATTA_COUPLING_CONSTANT")) CouplingChoice = HATTA_COUPLING_CONSTANT;
+        if (!strcmp(coupling, "FIXED_COUPLING")) CouplingChoice = FIXED_COUPLING;
+        if (!strcmp(coupling, "CORRELATED_NOISE_COUPLING")) CouplingChoice = CORRELATED_NOISE_COUPLING;
+        if (!strcmp(kernel, "LINEAR_KERNEL")) KernelChoice = LINEAR_KERNEL;
+        if (!strcmp(kernel, "SIN_KERNEL")) KernelChoice = SIN_KERNEL;
+        // optional keys, any order
+        char key[64], val[128];
+        while (fscanf(f, " %63[^= ] = %127s", key, val) == 2) {
+            if (!strcmp(key, "N")) Nx = Ny = atoi(val);
+            else if (!strcmp(key, "seed")) seed = (unsigned)strtoul(val, nullptr, 10);
+            else if (!strcmp(key, "lambda_L")) lambda_L = atof(val);
+            else if (!strcmp(key, "mu0_over_lambda")) mu0_over_lambda = atof(val);
+            else if (!strcmp(key, "engine")) fused = strcmp(val, "reference") != 0;
+            else fprintf(stderr, "config: unknown key %s ignored\n", key);
+        }
+        fclose(f);
+        return 1;
+    }
+};
+
+#endif

@@ -1,0 +1,246 @@
+// field.h -- lfield / gfield and the free functions of the Langevin step, the
+// surface of the reference's field.h (lfield/gfield field.h:101-299, the step
+// functions field.h:1892-2075 and 2116-2365, measurement field.h:535-591 and
+// 1375-1536, print field.h:2521-2547), over the HIP library's C ABI.
+//
+// A field lives in device memory (jw_field); `u` is a host copy that
+// to_host() fills and from_host() uploads.  All fields of a run share one
+// context (one lattice on one GPU), created by fftw2D::init2D().
+// Errors print the library's message and exit with status 3: the reference
+// prints and carries on (e.g. field.h:749-753), which hides failures.
+#ifndef JW_HOST_FIELD_H
+#define JW_HOST_FIELD_H
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "jimwlk.h"
+#include "config.h"
+#include "lattice.h"
+
+namespace jwhost {
+
+struct State {
+    jw_ctx* ctx = nullptr;
+    config* cnfg = nullptr;
+    jw_alpha alpha{6.0, 2.5, 0.2, 9.0};
+};
+
+inline State& state() {
+    static State s;
+    return s;
+}
+
+inline void check(int rc, const char* what) {
+    if (rc != JW_OK) {
+        fprintf(stderr, "%s failed (%d): %s\n", what, rc, jw_last_error());
+        exit(3);
+    }
+}
+
+inline jw_ctx* ctx() {
+    if (!state().ctx) {
+        fprintf(stderr, "fields need the lattice context: construct fftw2D and call init2D() first\n");
+        exit(3);
+    }
+    return state().ctx;
+}
+
+inline int coupling_code(Coupling c) { return (int)c; }
+
+}  // namespace jwhost
+
+template <class T, int t>
+class lfield {
+   public:
+    jw_field* f = nullptr;
+    int Nxl, Nyl;
+    std::vector<std::complex<T>> u;   // host copy (to_host / from_host)
+
+    lfield(int NNx, int NNy) : Nxl(NNx), Nyl(NNy) {
+        static_assert(t == 1 || t == 9, "fields have 1 or 9 components");
+        jwhost::check(jw_field_create(jwhost::ctx(), t, &f), "lfield allocation");
+    }
+    lfield(const lfield& in) : Nxl(in.Nxl), Nyl(in.Nyl) {
+        jwhost::check(jw_field_create(jwhost::ctx(), t, &f), "lfield allocation");
+        jwhost::check(jw_field_copy(f, in.f), "lfield copy");
+    }
+    virtual ~lfield() { jw_field_destroy(f); }
+
+    std::complex<T>* to_host() {
+        u.resize((size_t)t * Nxl * Nyl);
+        jwhost::check(jw_field_download(f, reinterpret_cast<double*>(u.data())), "download");
+        return u.data();
+    }
+    int from_host() {
+        jwhost::check(jw_field_upload(f, reinterpret_cast<const double*>(u.data())), "upload");
+        return 1;
+    }
+
+    int loc_pos(int x, int y) { return y + Nyl * x; }
+
+    int setToZero() {
+        jwhost::check(jw_set_to_zero(f), "setToZero");
+        return 1;
+    }
+    int setToUnit() {
+        jwhost::check(jw_set_to_unit(f), "setToUnit");
+        return 1;
+    }
+    lfield& operator=(const lfield& in) {
+        if (this != &in) jwhost::check(jw_field_copy(f, in.f), "operator=");
+        return *this;
+    }
+    // U <- U exp(i f)   (field.h:337-365)
+    lfield& operator*=(const lfield& g) {
+        if (this != &g) jwhost::check(jw_mul_exp(jwhost::ctx(), f, g.f), "operator*=");
+        return *this;
+    }
+    // one MV sheet of colour charge (field.h:683-757), charges from the context's noise source
+    int setMVModel(MV_class* mv) {
+        jwhost::check(jw_set_mv_model(jwhost::ctx(), f, mv->g_parameter, mv->mu_parameter, mv->Ny_parameter),
+                      "setMVModel");
+        return 1;
+    }
+    int solvePoisson(double mass, double g, momenta*) {
+        jwhost::check(jw_solve_poisson(jwhost::ctx(), f, mass, g), "solvePoisson");
+        return 1;
+    }
+    int trace(lfield<double, 1>* cc) {
+        jwhost::check(jw_trace(jwhost::ctx(), f, cc->f), "trace");
+        return 1;
+    }
+};
+
+template <class T, int t>
+class gfield : public lfield<T, t> {
+   public:
+    int Nxg, Nyg;
+    gfield(int NNx, int NNy) : lfield<T, t>(NNx, NNy), Nxg(NNx), Nyg(NNy) {}
+    // one rank holds the whole lattice: the gather is a device copy (field.h:535-591)
+    int allgather(lfield<T, t>* ulocal, mpi_class*) {
+        jwhost::check(jw_field_copy(this->f, ulocal->f), "allgather");
+        return 1;
+    }
+    int average_and_symmetrize() {
+        jwhost::check(jw_average_and_symmetrize(jwhost::ctx(), this->f), "average_and_symmetrize");
+        return 1;
+    }
+    int reduce(lfield<T, t>* sum, lfield<T, t>* err, mpi_class*) {
+        jwhost::check(jw_reduce(jwhost::ctx(), this->f, sum->f, err->f), "reduce");
+        return 1;
+    }
+};
+
+// fftw2D (mpi_fftw_class.h:113-352): unitary 2-D transforms of fields.  init2D
+// also creates the lattice context on device 0 and seeds the Philox stream.
+class fftw2D {
+    config* cnfg;
+
+   public:
+    explicit fftw2D(config* c) : cnfg(c) {}
+    ~fftw2D() {
+        jwhost::State& s = jwhost::state();
+        if (s.ctx) jw_destroy(s.ctx);
+        s.ctx = nullptr;
+    }
+    int init2D() {
+        jwhost::State& s = jwhost::state();
+        if (cnfg->Nx != cnfg->Ny) {
+            fprintf(stderr, "fftw2D: Nx == Ny required (%d x %d)\n", cnfg->Nx, cnfg->Ny);
+            exit(3);
+        }
+        if (!s.ctx) jwhost::check(jw_create(&s.ctx, cnfg->Nx, 0), "lattice context");
+        s.cnfg = cnfg;
+        s.alpha = jw_alpha{cnfg->lambda_L, cnfg->mu0_over_lambda, 0.2, 9.0};
+        jwhost::check(jw_noise_philox(s.ctx, cnfg->seed, 0, 0), "noise stream");
+        return 1;
+    }
+    template <class T, int t>
+    int execute2D(lfield<T, t>* f, int dir) {
+        jwhost::check(jw_execute2D(jwhost::ctx(), f->f, dir), "execute2D");
+        return 1;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// the step functions (field.h:1892-2075, 2116-2365)
+// ---------------------------------------------------------------------------
+template <class T, int t>
+int generate_gaussian(lfield<T, t>* xi_x, lfield<T, t>* xi_y, mpi_class*, config*) {
+    jwhost::check(jw_generate_gaussian(jwhost::ctx(), xi_x->f, xi_y->f), "generate_gaussian");
+    return 1;
+}
+
+template <class T, int t>
+int generate_gaussian_with_noise_coupling_constant(lfield<T, t>* xi_x, lfield<T, t>* xi_y, momenta*, mpi_class*,
+                                                    config*) {
+    jwhost::check(jw_generate_gaussian_with_noise_coupling_constant(jwhost::ctx(), xi_x->f, xi_y->f,
+                                                                    &jwhost::state().alpha),
+                  "generate_gaussian_with_noise_coupling_constant");
+    return 1;
+}
+
+template <class T, int t>
+int prepare_A_local(lfield<T, t>* A, lfield<T, t>* xi_x, lfield<T, t>* xi_y, momenta*, mpi_class*, Coupling p,
+                    Kernel kk) {
+    jwhost::check(jw_prepare_A_local(jwhost::ctx(), A->f, xi_x->f, xi_y->f, jwhost::coupling_code(p), (int)kk,
+                                     &jwhost::state().alpha),
+                  "prepare_A_local");
+    return 1;
+}
+
+template <class T, int t>
+int uxiulocal(lfield<T, t>* vx, lfield<T, t>* vy, lfield<T, t>* uf, lfield<T, t>* xi_x, lfield<T, t>* xi_y) {
+    jwhost::check(jw_uxiulocal(jwhost::ctx(), vx->f, vy->f, uf->f, xi_x->f, xi_y->f), "uxiulocal");
+    return 1;
+}
+
+template <class T, int t>
+int update_uf(lfield<T, t>* uf, lfield<T, t>* B, lfield<T, t>* A, double step) {
+    jwhost::check(jw_update_uf(jwhost::ctx(), uf->f, B->f, A->f, step), "update_uf");
+    return 1;
+}
+
+// The whole momentum-space step in the library's fused generator-basis engine
+// (same result as the per-function sequence above, to rounding).
+template <class T, int t>
+int evolve(lfield<T, t>* uf, config* cnfg, int nsteps) {
+    jwhost::check(jw_evolve(jwhost::ctx(), uf->f, jwhost::coupling_code(cnfg->CouplingChoice), (int)cnfg->KernelChoice,
+                            cnfg->step, &jwhost::state().alpha, nsteps),
+                  "evolve");
+    return 1;
+}
+
+// print (field.h:2521-2547): rows "meas kx ky sqrt(phat2) x*phat2*sum x*phat2*err"
+// for |kx - ky| <= 4, appended to <fileroot>_<Nx>_<Ny>_mpi<size>_r<rank>.dat
+template <class T, int t>
+int print(int measurement, lfield<T, t>* sum, lfield<T, t>* err, momenta* mom, double x, mpi_class* mpi,
+          const std::string& fileroot) {
+    char filename[512];
+    snprintf(filename, sizeof filename, "%s_%i_%i_mpi%i_r%i.dat", fileroot.c_str(), sum->Nxl, sum->Nyl,
+             mpi->getSize(), mpi->getRank());
+    FILE* f = fopen(filename, "a+");
+    if (!f) {
+        fprintf(stderr, "print: cannot open %s\n", filename);
+        return 0;
+    }
+    const std::complex<T>* s = sum->to_host();
+    const std::complex<T>* e = err->to_host();
+    for (int xx = 0; xx < sum->Nxl; xx++)
+        for (int yy = 0; yy < sum->Nyl; yy++) {
+            const int i = xx * sum->Nyl + yy;
+            if (fabs((double)(xx - yy)) <= 4)
+                fprintf(f, "%i %i %i \t %f %e %e\n", measurement, xx, yy, sqrt(mom->phat2(i)),
+                        x * (mom->phat2(i)) * (s[i * t].real()), x * (mom->phat2(i)) * (e[i * t].real()));
+        }
+    fclose(f);
+    return 1;
+}
+
+#endif

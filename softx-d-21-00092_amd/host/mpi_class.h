@@ -1,0 +1,5 @@
+// mpi_class.h -- the reference's header name for mpi_class; the class lives in lattice.h.
+#ifndef JW_HOST_MPI_CLASS_H
+#define JW_HOST_MPI_CLASS_H
+#include "lattice.h"
+#endif

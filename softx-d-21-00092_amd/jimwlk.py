@@ -80,6 +80,7 @@ def lib():
         L.jw_kernel_time.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]
         L.jw_su3_exp_gen.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, ctypes.c_int]
         L.jw_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_set_fused.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -226,9 +227,12 @@ class Context:
     def timing(self, enable=True):
         _check(lib().jw_timing(self.h, 1 if enable else 0))
 
+    def set_fused(self, enable=True):
+        _check(lib().jw_set_fused(self.h, 1 if enable else 0))
+
     def kernel_times(self):
         out = {}
-        for k in range(9):
+        for k in range(12):
             ms, n = ctypes.c_double(), ctypes.c_long()
             _check(lib().jw_kernel_time(self.h, k, ctypes.byref(ms), ctypes.byref(n)))
             if n.value:

@@ -1,0 +1,94 @@
+"""The C++ driver (softx-d-21-00092_amd/main_optimized, the reference's
+main_optimized structure over the C ABI) end to end on the GPU: MV initial
+condition, Langevin steps, measurement and the reference's output file
+(field.h:2521-2547), for both engines, against the oracle fed the same Philox
+stream.  The output file prints 7 significant digits, which bounds the check."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(REPO, "softx-d-21-00092_amd", "main_optimized")
+
+
+def write_cfg(path, N, stat, mass, Ny, steps, meas, step, coupling, engine, seed=11):
+    path.write_text(f"""stat = {stat}
+mu = 30.72
+mass = {mass}
+elementaryWilsonLines = {Ny}
+file_name = out_{engine}
+langevin_steps = {steps}
+measurements = {meas}
+step = {step}
+EvolutionChoice = MOMENTUM_EVOLUTION
+CouplingChoice = {coupling}
+KernelChoice = SIN_KERNEL
+N = {N}
+seed = {seed}
+engine = {engine}
+""")
+
+
+def read_out(path):
+    rows = [l.split() for l in open(path) if l.strip()]
+    return np.array([[float(v) for v in r] for r in rows])
+
+
+def oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, coupling, seed=11):
+    """main_optimized.cpp:159-355 through the oracle with the driver's Philox stream."""
+    mu = float(np.float32(30.72)) / N
+    m = float(np.float32(mass))
+    ds = float(np.float32(step))
+    every = max(steps // meas, 1)
+    V = N * N
+    sums = [np.zeros(V) for _ in range(meas)]
+    errs = [np.zeros(V) for _ in range(meas)]
+    for s in range(stat):
+        z = np.concatenate([orc.philox_normals(N, seed, s, k, jw.TAG_MV, 8) for k in range(Ny)])
+        U = orc.mv_init(N, z, Ny, 1.0, mu, m)        # Poisson mass m g^2 mu inside
+        for l in range(steps):
+            zl = orc.philox_normals(N, seed, s, Ny + l, jw.TAG_LANGEVIN, 16)
+            U = orc.langevin_step(U, zl, N, coupling, 1, ds)
+            if l % every == 0 and l // every < meas:
+                c = orc.measure_corr(U, N)[:, 0].real
+                sums[l // every] += c
+                errs[l // every] += c * c
+    ph, _, _ = orc.momenta(N)
+    x = 1.0 / 3.0 / stat
+    rows = []
+    for xx in range(N):
+        for yy in range(N):
+            if abs(xx - yy) <= 4:
+                i = xx * N + yy
+                rows.append([meas - 1, xx, yy, np.sqrt(ph[i]), x * ph[i] * sums[-1][i], x * ph[i] * errs[-1][i]])
+    return np.array(rows)
+
+
+@pytest.mark.parametrize("coupling,code", [("NOISE_COUPLING_CONSTANT", 1), ("FIXED_COUPLING", 3)])
+def test_driver_end_to_end(jw, orc, tmp_path, coupling, code):
+    assert os.path.exists(DRIVER), "driver not built"
+    N, stat, mass, Ny, steps, meas, step = 32, 2, 2.0, 6, 8, 4, 0.0004
+    outs = {}
+    for engine in ("fused", "reference"):
+        cfg = tmp_path / f"cfg_{engine}"
+        write_cfg(cfg, N, stat, mass, Ny, steps, meas, step, coupling, engine)
+        r = subprocess.run([DRIVER, "1", "1", str(cfg)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-2000:]
+        assert "Evolution time" in r.stdout
+        outs[engine] = read_out(tmp_path / f"out_{engine}_{N}_{N}_mpi1_r0.dat")
+    ref = oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, code)
+    for engine, got in outs.items():
+        assert got.shape == ref.shape
+        assert np.array_equal(got[:, :3], ref[:, :3])
+        np.testing.assert_allclose(got[:, 3:], ref[:, 3:], rtol=2e-6, atol=1e-12)
+
+
+def test_driver_rejects_multi_rank_grid(tmp_path):
+    cfg = tmp_path / "cfg"
+    write_cfg(cfg, 16, 1, 1.0, 1, 1, 1, 0.0004, "FIXED_COUPLING", "fused")
+    r = subprocess.run([DRIVER, "2", "1", str(cfg)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "1 x 1" in r.stderr

@@ -258,6 +258,27 @@ def test_evolve_one_step(jw, orc, ctx64, coupling, kernel):
     assert frob_max(ref, U0) > 1e-3
 
 
+@pytest.mark.parametrize("N", [64, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("coupling", [3, 1, 0, 4])
+def test_fused_matches_unfused(jw, orc, N, coupling):
+    """The fused row/column passes against the unfused kernel sequence, Philox noise."""
+    c = jw.Context(N)
+    U0 = c.field()
+    c.philox(5, 1, 0)
+    c.mv_init(U0, 2, 1.0, 0.25, 1.0)
+    ref = U0.download()
+    results = []
+    for fused in (0, 1):
+        jw.lib().jw_set_fused(c.h, fused)
+        U = c.field(9, ref)
+        c.philox(5, 1, 100)
+        c.evolve(U, coupling, 1, 1e-4, nsteps=2)
+        results.append(U.download())
+    assert frob_max(results[0], results[1]) < 1e-13
+    assert frob_max(results[0], ref) > 1e-3
+    c.close()
+
+
 def test_evolve_zero_noise_is_identity(jw, orc, ctx64):
     N = 64
     U0 = random_su3(orc, N * N)
