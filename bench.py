@@ -93,13 +93,13 @@ def main():
     # step (complex planes of 16 B each way; U is 9 complex planes):
     #   row_noise_fwd  W 8 planes                                  128
     #   col_kmult      A: R 8 W 4 (+W 8 correlated), B: R 8 W 4    384 (512)
+    #   fft_rows       correlated only: inverse rows of xi~ R 8 W 8  (256)
     #   row_uxiu_fwd   R U 9 (+ xi~ 8 correlated), W 8             272 (400)
-    #   fft_rows       inverse rows of A,B (R 8 W 8) (+ xi~)       256 (512)
-    #   update         R U 9 + A 4 + B 4, W U 9                    416
+    #   row_update     R A 4 + B 4 + U 9, W U 9                    416
     kind = "correlated" if a.coupling == "correlated" else "white"
     corr_ = kind == "correlated"
     alg = {"row_noise_fwd": 128, "col_kmult": 512 if corr_ else 384, "row_uxiu_fwd": 400 if corr_ else 272,
-           "fft_rows": 512 if corr_ else 256, "update": 416}
+           "fft_rows": 256, "row_update": 416}
     V = N * N
     nk = min(a.steps, 10)
     per_step_ms = {k: ms * n / nk for k, (ms, n) in kt.items()}

@@ -553,7 +553,7 @@ int jw_saturation_scale(int nbins, const double* rbar, const double* Sb, const d
 // ---------------------------------------------------------------- instrumentation
 static const char* kNames[] = {"fft_rows", "fft_cols", "gen_noise", "kernel_mult", "uxiu",
                                "update", "mv", "observables", "reference_functions",
-                               "row_noise_fwd", "col_kmult", "row_uxiu_fwd"};
+                               "row_noise_fwd", "col_kmult", "row_uxiu_fwd", "row_update"};
 
 int jw_set_fused(jw_ctx* c, int enable) {
     if (!c) return fail(JW_ERR_ARG, "null context");

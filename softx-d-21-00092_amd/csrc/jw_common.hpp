@@ -13,7 +13,7 @@ namespace jw {
 // timing of kernel classes with HIP events on the context stream
 // ---------------------------------------------------------------------------
 enum KClass { K_FFT_ROWS = 0, K_FFT_COLS, K_GEN_NOISE, K_KMULT, K_UXIU, K_UPDATE, K_MV, K_OBS, K_DROPIN,
-              K_ROW_NOISE, K_COL_KMULT, K_ROW_UXIU, K_NCLASS };
+              K_ROW_NOISE, K_COL_KMULT, K_ROW_UXIU, K_ROW_UPDATE, K_NCLASS };
 
 struct KTimer {
     jw_ctx* c;

@@ -237,6 +237,62 @@ __global__ __launch_bounds__(ColE<N>::NT) void k_col_kmult(cplx* __restrict__ Z,
 }
 
 // ---------------------------------------------------------------------------
+// row pass 3: inverse row FFTs of the A and B column-inverse spectra and the
+// update U <- exp(i s B) U exp(-i s A) (field.h:1966-1996) in one pass.
+// A's coefficients are kept in registers, B's stay in LDS.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(RowE<N>::NT) void k_row_update(cplx* __restrict__ U, const cplx* __restrict__ ZA,
+                                                           const cplx* __restrict__ ZB, size_t V, double ss,
+                                                           const cplx* __restrict__ tw) {
+    constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
+    __shared__ cplx lds[4 * N];
+    const int tid = threadIdx.x;
+    const size_t row0 = (size_t)blockIdx.x * N;
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int q = 0; q < SPT; q++) lds[p * N + tid + q * NT] = ZA[p * V + row0 + tid + q * NT];
+    __syncthreads();
+    lds_fft<N, 4, 1, N, NT, +1>(lds, tw, tid);
+    double ha[SPT][8];
+#pragma unroll
+    for (int q = 0; q < SPT; q++)
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const cplx a = lds[p * N + tid + q * NT];
+            ha[q][2 * p] = a.x;
+            ha[q][2 * p + 1] = a.y;
+        }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int q = 0; q < SPT; q++) lds[p * N + tid + q * NT] = ZB[p * V + row0 + tid + q * NT];
+    __syncthreads();
+    lds_fft<N, 4, 1, N, NT, +1>(lds, tw, tid);
+#pragma unroll
+    for (int q = 0; q < SPT; q++) {
+        const int y = tid + q * NT;
+        const size_t i = row0 + y;
+        double hb[8];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const cplx b = lds[p * N + y];
+            hb[2 * p] = b.x;
+            hb[2 * p + 1] = b.y;
+        }
+        m3 u;
+#pragma unroll
+        for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+        const m3 w = mmul(exp_i_gen(hb, ss), u);
+        const m3 r = mmul(w, exp_i_gen(ha[q], -ss));
+#pragma unroll
+        for (int k = 0; k < 9; k++) U[k * V + i] = r.m[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <int N>
@@ -274,16 +330,20 @@ static hipError_t step_fused_n(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA,
                            V, sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    // ZA and ZB are adjacent: one inverse row pass over their 8 planes
-    if ((e = fft_rows(c, ZA, 8, 0, 1.0)) != hipSuccess) return e;
-    return update_packed(c, U, ZA, ZB, sqrt_step);
+    {
+        KTimer t(c, kcls[3]);
+        hipLaunchKernelGGL((k_row_update<N>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, U, ZA, ZB, V, sqrt_step,
+                           c->tw);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 bool fused_supported(int N) { return N >= 64 && N <= 2048; }
 
 hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
                       const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr, double sqrt_step) {
-    int kcls[3] = {K_ROW_NOISE, K_COL_KMULT, K_ROW_UXIU};
+    int kcls[4] = {K_ROW_NOISE, K_COL_KMULT, K_ROW_UXIU, K_ROW_UPDATE};
     switch (c->N) {
         case 64: return step_fused_n<64>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
         case 128: return step_fused_n<128>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);

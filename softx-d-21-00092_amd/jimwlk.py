@@ -232,7 +232,7 @@ class Context:
 
     def kernel_times(self):
         out = {}
-        for k in range(12):
+        for k in range(13):
             ms, n = ctypes.c_double(), ctypes.c_long()
             _check(lib().jw_kernel_time(self.h, k, ctypes.byref(ms), ctypes.byref(n)))
             if n.value:
