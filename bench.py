@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--N", type=int, default=2048)
     p.add_argument("--coupling", default="correlated", choices=["correlated", "fixed", "noise", "sqrt"])
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--calibrate", action="store_true",
+                   help="also run a plain copy of 8 planes (k_calibration_copy) to scale traffic counters")
     p.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     return p.parse_args()
 
@@ -75,6 +77,9 @@ def main():
     t_mv = time.perf_counter() - t0
     print(f"bench: N={N} MV init (50 sheets) {t_mv * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
+    if a.calibrate:
+        ctx.calibration_copy(8)
+        ctx.sync()
     ctx.evolve(U, coupling, jw.SIN_KERNEL, step, al, nsteps=a.warmup)
     ctx.sync()
     t0 = time.perf_counter()

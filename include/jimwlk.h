@@ -171,6 +171,10 @@ int jw_timing(jw_ctx* ctx, int enable);
 /* Average duration (ms) and call count of kernel class `which` (see jw_kernel_name). */
 int jw_kernel_time(const jw_ctx* ctx, int which, double* avg_ms, long* calls);
 const char* jw_kernel_name(int which);
+/* A plain device copy of nplanes complex planes (16 V nplanes bytes read and
+ * as many written) in the context's workspace, kernel k_calibration_copy: the
+ * reference size for scaling hardware traffic counters. */
+int jw_calibration_copy(jw_ctx* ctx, int nplanes);
 /* exp(i s H(h)) for n generator vectors h (8 reals each) -> n 3x3 matrices (18 reals). */
 int jw_su3_exp_gen(jw_ctx* ctx, const double* host_h8, double s, double* host_out18, int n);
 

@@ -581,6 +581,14 @@ const char* jw_kernel_name(int which) {
     return kNames[which];
 }
 
+int jw_calibration_copy(jw_ctx* c, int nplanes) {
+    if (!c || nplanes < 1) return fail(JW_ERR_ARG, "bad argument");
+    if (int e = ensure_ws(c, (size_t)2 * nplanes)) return e;
+    const size_t n = (size_t)nplanes * c->V;
+    HIP_CHECK(jw::calibration_copy(c, c->ws, c->ws + n, n));
+    return JW_OK;
+}
+
 int jw_su3_exp_gen(jw_ctx* c, const double* h8, double s, double* out18, int n) {
     if (!c || !h8 || !out18 || n < 0) return fail(JW_ERR_ARG, "bad argument");
     if (n == 0) return JW_OK;

@@ -1,7 +1,7 @@
 // jw_device.hpp -- device-side building blocks of the MI355X JIMWLK solver:
 // complex FP64 arithmetic, SU(3) / su(3) algebra in the reference's colour basis,
-// the SU(3) exponential (closed-form eigenvalues + Newton divided differences),
-// the Philox4x32-10 noise stream and the LDS Stockham FFT.
+// the SU(3) exponential (Cayley-Hamilton with series coefficients, scaling and
+// squaring), the Philox4x32-10 noise stream and the LDS Stockham FFT.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -81,45 +81,33 @@ __device__ __forceinline__ void gen_from_upper(const m3& M, double h[8]) {
 
 // ---------------------------------------------------------------------------
 // exp(i X) for X Hermitian traceless 3x3 (Cayley-Hamilton form).
-//   eigenvalues a >= b >= c of X from the depressed cubic l^3 - c1 l - c0 = 0,
-//   c1 = tr(X^2)/2, c0 = det X (trigonometric solution);
-//   exp(iX) = e^{ia} 1 + [a,b] (X - a) + [a,b,c] (X - a)(X - b)   (Newton form)
-//   with the divided differences of e^{ix}:
-//   [a,b] = i e^{i(a+b)/2} sinc((a-b)/2),  [a,b,c] = ([a,b]-[b,c])/(a-c).
-// All branches are smooth; X = 0 gives exactly 1.
+// With X^3 = c1 X + c0 (c1 = tr(X^2)/2, c0 = det X), X^n = a_n + b_n X + d_n X^2
+// where (a, b, d)_{n+1} = (c0 d_n, a_n + c1 d_n, b_n); hence
+//   exp(iX) = f0 + f1 X + f2 X^2,  f = sum_n i^n (a_n, b_n, d_n) / n!.
+// The sum runs to 24 terms for spectral radius rho <= 2 sqrt(c1/3) <= 1.6
+// (c1 <= 1.92): the remainder is below rho^25/25! ~ 1e-20.  Larger X are
+// scaled by 2^-k (exact) into that range and the result squared k times.
+// X = 0 gives exactly 1.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double sinc_(double x) {
-    const double x2 = x * x;
-    if (fabs(x) < 1e-4) return 1.0 - x2 * (1.0 / 6.0) * (1.0 - x2 * (1.0 / 20.0));
-    return sin(x) / x;
-}
+#define JW_EXP_SERIES_C1 1.92
+#define JW_EXP_SERIES_TERMS 24
 
-__device__ __forceinline__ cplx expi(double t) {
-    double s, c;
-    sincos(t, &s, &c);
-    return mk(c, s);
-}
-
-// [a,b] of e^{ix}
-__device__ __forceinline__ cplx divdiff1(double a, double b) {
-    const cplx e = expi(0.5 * (a + b));
-    return cmuli(cscale(e, sinc_(0.5 * (a - b))));
-}
-
-// X given by generator coefficients h scaled by s; returns E = exp(i s H(h)).
 __device__ __forceinline__ m3 exp_i_gen(const double h[8], double s) {
-    m3 X = herm_from_gen(h, s);
     double c1 = 0.0;
 #pragma unroll
     for (int a = 0; a < 8; a++) c1 += h[a] * h[a];
     c1 *= s * s;
     m3 E;
-    if (!(c1 > 0.0)) {   // X == 0 (or NaN input: propagate through the sum below)
+    if (!(c1 > 0.0)) {   // X == 0 (or NaN input: propagate it)
 #pragma unroll
         for (int k = 0; k < 9; k++) E.m[k] = mk((k % 4 == 0) ? 1.0 : 0.0, 0.0);
         if (c1 != c1) E.m[0] = mk(c1, c1);
         return E;
     }
+    // squarings needed: c1 / 4^k <= JW_EXP_SERIES_C1
+    int k = 0;
+    while (c1 > JW_EXP_SERIES_C1 && k < 60) { c1 *= 0.25; k++; }
+    const m3 X = herm_from_gen(h, ldexp(s, -k));
     // det X (Hermitian: real)
     const double d0 = X.m[0].x, d1 = X.m[4].x, d2 = X.m[8].x;
     const cplx x = X.m[1], y = X.m[2], z = X.m[5];
@@ -127,36 +115,28 @@ __device__ __forceinline__ m3 exp_i_gen(const double h[8], double s) {
     const double c0 = d0 * d1 * d2 + 2.0 * (xz.x * y.x + xz.y * y.y)
                       - d0 * (z.x * z.x + z.y * z.y) - d1 * (y.x * y.x + y.y * y.y)
                       - d2 * (x.x * x.x + x.y * x.y);
-    const double r = sqrt(c1 * (1.0 / 3.0));
-    const double r3x2 = 2.0 * r * r * r;
-    const double disc = fmax(0.0, (r3x2 - c0) * (r3x2 + c0));
-    const double th = atan2(sqrt(disc), c0) * (1.0 / 3.0);
-    double sth, cth;
-    sincos(th, &sth, &cth);
-    const double a = 2.0 * r * cth;
-    const double b = r * (-cth + JW_SQRT3 * sth);
-    const double c = r * (-cth - JW_SQRT3 * sth);
-    const cplx ea = expi(a);
-    const cplx dab = divdiff1(a, b);
-    cplx dabc;
-    if (a - c > 1e-3) {
-        const cplx dbc = divdiff1(b, c);
-        dabc = cscale(csub(dab, dbc), 1.0 / (a - c));
-    } else {
-        // all eigenvalues within 1e-3 of 0: second divided difference of e^{ix}
-        // by its Taylor series (a + b + c = 0 removes the first-order term)
-        dabc = mk(-0.5 + (a * a + b * b + c * c + a * b + b * c + c * a) * (1.0 / 24.0), 0.0);
-    }
-    // Y = X - a, Z = X - b; YZ Hermitian
-    m3 Y = X, Z = X;
-    Y.m[0].x -= a; Y.m[4].x -= a; Y.m[8].x -= a;
-    Z.m[0].x -= b; Z.m[4].x -= b; Z.m[8].x -= b;
-    const m3 YZ = mmul(Y, Z);
+    // scaled coefficients A = a_n/n!, B = b_n/n!, D = d_n/n!; n = 0
+    double A = 1.0, B = 0.0, D = 0.0;
+    double f0r = 1.0, f0i = 0.0, f1r = 0.0, f1i = 0.0, f2r = 0.0, f2i = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; k++) E.m[k] = cadd(cmul(dab, Y.m[k]), cmul(dabc, YZ.m[k]));
-    E.m[0] = cadd(E.m[0], ea);
-    E.m[4] = cadd(E.m[4], ea);
-    E.m[8] = cadd(E.m[8], ea);
+    for (int n = 1; n <= JW_EXP_SERIES_TERMS; n++) {
+        const double inv = 1.0 / n;
+        const double An = c0 * D * inv, Bn = (A + c1 * D) * inv, Dn = B * inv;
+        A = An; B = Bn; D = Dn;
+        // i^n: n%4 = 0 -> +1, 1 -> +i, 2 -> -1, 3 -> -i
+        if ((n & 3) == 0) { f0r += A; f1r += B; f2r += D; }
+        else if ((n & 3) == 1) { f0i += A; f1i += B; f2i += D; }
+        else if ((n & 3) == 2) { f0r -= A; f1r -= B; f2r -= D; }
+        else { f0i -= A; f1i -= B; f2i -= D; }
+    }
+    const cplx f0 = mk(f0r, f0i), f1 = mk(f1r, f1i), f2 = mk(f2r, f2i);
+    const m3 X2 = mmul(X, X);
+#pragma unroll
+    for (int q = 0; q < 9; q++) E.m[q] = cadd(cmul(f1, X.m[q]), cmul(f2, X2.m[q]));
+    E.m[0] = cadd(E.m[0], f0);
+    E.m[4] = cadd(E.m[4], f0);
+    E.m[8] = cadd(E.m[8], f0);
+    for (int j = 0; j < k; j++) E = mmul(E, E);
     return E;
 }
 
@@ -168,10 +148,11 @@ __device__ __forceinline__ m3 exp_i_gen(const double h[8], double s) {
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; r++) {
-        const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
-        const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
-        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
-        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        // one 32x32 -> 64 bit product each (v_mad_u64_u32)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0; c[1] = (uint32_t)p1; c[2] = n2; c[3] = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -237,17 +218,32 @@ __device__ __forceinline__ void dft_small(cplx* v) {
 // (k = j mod P) from inputs x[j + r N/R] to outputs x[(j-k) R + k + r P],
 // after twiddling input r by w_N^{r k N/(P R)}.  Every thread reads all its
 // inputs, the block synchronises, then writes, so one buffer suffices.
-// NSEQ sequences of length N; element n of sequence s at buf[n*ES + s*SS].
+// NSEQ sequences of length N; element n of sequence s sits at
+// lpos<ES, SS, PS>(n, s) = (n + (n >> PS)) ES + s SS: one padding slot every
+// 2^PS elements (PS = 0: none) keeps the strided Stockham writes off common
+// LDS banks (PS = 4 for rows, ES = 1; PS = 3 for column tiles, ES > 1).
 // tw[m] = exp(-2 pi i m / N).  NT threads take part.
 // ---------------------------------------------------------------------------
+template <int ES, int SS, int PS>
+__device__ __forceinline__ int lpos(int n, int s) {
+    return (n + (PS ? (n >> PS) : 0)) * ES + s * SS;
+}
+
+// padded length of N elements
+template <int N, int PS>
+struct padded { static constexpr int value = N + (PS ? (N >> PS) : 0); };
+
 template <int REM>
 struct radix_of { static constexpr int value = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2); };
 
-template <int N, int P, int R, int NSEQ, int ES, int SS, int NT, int DIR>
+template <int N, int P, int R, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
 __device__ __forceinline__ void fft_stage(cplx* buf, const cplx* __restrict__ tw, int tid) {
     constexpr int S = N / R;
     constexpr int NB = S * NSEQ;
     constexpr int PER = (NB + NT - 1) / NT;
+    // With power-of-two N, P, R and padding, lpos(j + r S) = lpos(j) + off(r S) and
+    // lpos(b + r P) = lpos(b) + off(r P), off(m) = (m + (m >> PS)) ES: one computed
+    // base per butterfly, compile-time offsets per input/output.
     cplx v[PER][R];
     int jj[PER], ss[PER];
 #pragma unroll
@@ -259,8 +255,9 @@ __device__ __forceinline__ void fft_stage(cplx* buf, const cplx* __restrict__ tw
         jj[q] = j;
         ss[q] = s;
         if (NB % NT == 0 || b < NB) {
+            const cplx* rb = buf + lpos<ES, SS, PS>(j, s);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[q][r] = buf[(j + r * S) * ES + s * SS];
+            for (int r = 0; r < R; r++) v[q][r] = rb[lpos<ES, 0, PS>(r * S, 0)];
         }
     }
     __syncthreads();
@@ -280,29 +277,30 @@ __device__ __forceinline__ void fft_stage(cplx* buf, const cplx* __restrict__ tw
                 }
             }
             dft_small<R, DIR>(v[q]);
+            cplx* wb = buf + lpos<ES, SS, PS>((j - k) * R + k, s);
 #pragma unroll
-            for (int r = 0; r < R; r++) buf[((j - k) * R + k + r * P) * ES + s * SS] = v[q][r];
+            for (int r = 0; r < R; r++) wb[lpos<ES, 0, PS>(r * P, 0)] = v[q][r];
         }
     }
     __syncthreads();
 }
 
-template <int N, int P, int NSEQ, int ES, int SS, int NT, int DIR>
+template <int N, int P, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
 struct fft_stages {
     static __device__ __forceinline__ void run(cplx* buf, const cplx* __restrict__ tw, int tid) {
         constexpr int R = radix_of<N / P>::value;
-        fft_stage<N, P, R, NSEQ, ES, SS, NT, DIR>(buf, tw, tid);
-        fft_stages<N, P * R, NSEQ, ES, SS, NT, DIR>::run(buf, tw, tid);
+        fft_stage<N, P, R, NSEQ, ES, SS, NT, DIR, PS>(buf, tw, tid);
+        fft_stages<N, P * R, NSEQ, ES, SS, NT, DIR, PS>::run(buf, tw, tid);
     }
 };
-template <int N, int NSEQ, int ES, int SS, int NT, int DIR>
-struct fft_stages<N, N, NSEQ, ES, SS, NT, DIR> {
+template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
+struct fft_stages<N, N, NSEQ, ES, SS, NT, DIR, PS> {
     static __device__ __forceinline__ void run(cplx*, const cplx* __restrict__, int) {}
 };
 
-template <int N, int NSEQ, int ES, int SS, int NT, int DIR>
+template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS = 0>
 __device__ __forceinline__ void lds_fft(cplx* buf, const cplx* __restrict__ tw, int tid) {
-    fft_stages<N, 1, NSEQ, ES, SS, NT, DIR>::run(buf, tw, tid);
+    fft_stages<N, 1, NSEQ, ES, SS, NT, DIR, PS>::run(buf, tw, tid);
 }
 
 }  // namespace jw

@@ -5,11 +5,10 @@
 //   row_noise_fwd  : Philox/injected noise -> xi planes, forward row FFT    W 8
 //   col_kmult (A)  : forward column FFT, kernel multiply, inverse column FFT R 8 W 4
 //                    (correlated noise: also writes F_col^-1[sqrt(alpha) F xi])
-//   [rows^-1 xi~]  : correlated noise only: inverse row FFT of xi~          R 8 W 8
-//   row_uxiu_fwd   : V = U xi U^dagger, forward row FFT                     R 9(+8) W 8
+//   row_uxiu       : V = U xi U^dagger, forward row FFT                      R 9 W 8
+//                    (correlated noise: inverse row FFT of xi~ first)     (R 17 W 8)
 //   col_kmult (B)  : as for A                                                R 8 W 4
-//   rows^-1 A,B    : inverse row FFTs of the 8 A/B planes                    R 8 W 8
-//   update         : U <- exp(i s B) U exp(-i s A)                           R 17 W 9
+//   row_update     : inverse row FFTs of A, B; U <- exp(i s B) U exp(-i s A) R 17 W 9
 // (units: complex planes of V doubles pairs, 16 B per site each).
 //
 // Row passes: one workgroup per lattice row, four planes at a time in LDS.
@@ -26,7 +25,8 @@ struct RowE {
     static_assert(SPT >= 1 && SPT * NT == N, "row pass needs N a multiple of its thread count");
 };
 // Column passes hold the X and Y tiles of one W-column strip together in LDS
-// (2 W N complex values, 128 KB at most): W = min(16, 4096 / N).
+// (2 W N complex values plus one padding row in eight, 144 KB at most):
+// W = min(16, 4096 / N).
 template <int N>
 struct ColE {
     static constexpr int W = (4096 / N > 16) ? 16 : 4096 / N;
@@ -51,7 +51,8 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_noise_fwd(cplx* __restrict_
                                                               const double* __restrict__ sa_pos,
                                                               const cplx* __restrict__ tw) {
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
-    __shared__ cplx lds[4 * N];
+    constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
+    __shared__ cplx lds[4 * PN];
     const int tid = threadIdx.x;
     const size_t row0 = (size_t)blockIdx.x * N;
     double m[SPT][8];
@@ -62,27 +63,27 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_noise_fwd(cplx* __restrict_
         normals16(ns, row0 + y, z);
         const double s = sa_pos ? sa_pos[row0 + y] : 1.0;
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[p * N + y] = mk(z[4 * p] * s, z[4 * p + 2] * s);
+        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mk(z[4 * p] * s, z[4 * p + 2] * s);
 #pragma unroll
         for (int k = 0; k < 8; k++) m[q][k] = z[2 * k + 1] * s;
     }
     __syncthreads();
-    lds_fft<N, 4, 1, N, NT, -1>(lds, tw, tid);
+    lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Xi[p * V + row0 + tid + q * NT] = lds[p * N + tid + q * NT];
+        for (int q = 0; q < SPT; q++) Xi[p * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < SPT; q++)
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[p * N + tid + q * NT] = mk(m[q][2 * p], m[q][2 * p + 1]);
+        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = mk(m[q][2 * p], m[q][2 * p + 1]);
     __syncthreads();
-    lds_fft<N, 4, 1, N, NT, -1>(lds, tw, tid);
+    lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Xi[(4 + p) * V + row0 + tid + q * NT] = lds[p * N + tid + q * NT];
+        for (int q = 0; q < SPT; q++) Xi[(4 + p) * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
 }
 
 // ---------------------------------------------------------------------------
@@ -95,7 +96,8 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(cplx* __restrict__
                                                              const double* __restrict__ sa_pos,
                                                              const cplx* __restrict__ tw) {
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
-    __shared__ cplx lds[4 * N];
+    constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
+    __shared__ cplx lds[4 * PN];
     const int tid = threadIdx.x;
     const size_t row0 = (size_t)blockIdx.x * N;
     double vy[SPT][8];
@@ -125,25 +127,81 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(cplx* __restrict__
         uxiu_gen(u, hx, vx);
         uxiu_gen(u, hy, vy[q]);
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[p * N + y] = mk(vx[2 * p], vx[2 * p + 1]);
+        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mk(vx[2 * p], vx[2 * p + 1]);
     }
     __syncthreads();
-    lds_fft<N, 4, 1, N, NT, -1>(lds, tw, tid);
+    lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Vv[p * V + row0 + tid + q * NT] = lds[p * N + tid + q * NT];
+        for (int q = 0; q < SPT; q++) Vv[p * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < SPT; q++)
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[p * N + tid + q * NT] = mk(vy[q][2 * p], vy[q][2 * p + 1]);
+        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = mk(vy[q][2 * p], vy[q][2 * p + 1]);
     __syncthreads();
-    lds_fft<N, 4, 1, N, NT, -1>(lds, tw, tid);
+    lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Vv[(4 + p) * V + row0 + tid + q * NT] = lds[p * N + tid + q * NT];
+        for (int q = 0; q < SPT; q++) Vv[(4 + p) * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+}
+
+// ---------------------------------------------------------------------------
+// row pass 2, correlated noise: reads the column-inverse spectra of xi~,
+// finishes their inverse transform along the row, and forms V_i = U xi~_i U^dagger
+// in two batches (i = x, then y).  In each batch a site's V coefficients
+// overwrite its own xi~ slots in LDS before the forward row FFT; U is read once
+// per batch (the second read is served by the caches).
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_corr(cplx* __restrict__ Vv, const cplx* __restrict__ U,
+                                                              const cplx* __restrict__ Xs, size_t V,
+                                                              const cplx* __restrict__ tw) {
+    constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
+    constexpr int PN = padded<N, 4>::value;
+    __shared__ cplx lds[4 * PN];
+    const int tid = threadIdx.x;
+    const size_t row0 = (size_t)blockIdx.x * N;
+#pragma unroll 1
+    for (int half = 0; half < 2; half++) {
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int q = 0; q < SPT; q++)
+                lds[lpos<1, PN, 4>(tid + q * NT, p)] = Xs[(4 * half + p) * V + row0 + tid + q * NT];
+        __syncthreads();
+        lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
+        // one site at a time: keeps a single U (9 complex) live
+#pragma unroll 1
+        for (int q = 0; q < SPT; q++) {
+            const int y = tid + q * NT;
+            const size_t i = row0 + y;
+            double h[8];
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const cplx a = lds[lpos<1, PN, 4>(y, p)];
+                h[2 * p] = a.x;
+                h[2 * p + 1] = a.y;
+            }
+            m3 u;
+#pragma unroll
+            for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+            double v[8];
+            uxiu_gen(u, h, v);
+#pragma unroll
+            for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mk(v[2 * p], v[2 * p + 1]);
+        }
+        __syncthreads();
+        lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int q = 0; q < SPT; q++)
+                Vv[(4 * half + p) * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -164,7 +222,7 @@ __global__ __launch_bounds__(ColE<N>::NT) void k_col_kmult(cplx* __restrict__ Z,
                                                           const cplx* __restrict__ tw) {
     constexpr int W = ColE<N>::W, NT = ColE<N>::NT, PER = ColE<N>::PER, PAIRS = ColE<N>::PAIRS;
     constexpr int S2 = 2 * W;
-    __shared__ cplx lds[2 * W * N];
+    __shared__ cplx lds[padded<N, 3>::value * S2];
     const int tid = threadIdx.x;
     const int strip = xcd_strip(blockIdx.x, N / W);
     const int col0 = strip * W;
@@ -176,18 +234,18 @@ __global__ __launch_bounds__(ColE<N>::NT) void k_col_kmult(cplx* __restrict__ Z,
         const int e = tid + q * NT;
         const int x = e / S2, s = e % S2;
         const size_t g = pofs + (size_t)x * N + col0 + (s % W);
-        lds[e] = s < W ? X[g] : Y[g];
+        lds[lpos<S2, 1, 3>(x, s)] = s < W ? X[g] : Y[g];
     }
     __syncthreads();
-    lds_fft<N, S2, S2, 1, NT, -1>(lds, tw, tid);
-    // pair (x, c): X^ at lds[x*2W + c], Y^ at lds[x*2W + W + c]
+    lds_fft<N, S2, S2, 1, NT, -1, 3>(lds, tw, tid);
+    // pair (x, c): X^ in slot c of LDS row x, Y^ in slot W + c
     cplx zreg[CORR ? PAIRS : 1];
 #pragma unroll
     for (int q = 0; q < PAIRS; q++) {
         const int pr = tid + q * NT;
         const int kx = pr / W, c = pr % W, ky = col0 + c;
         const size_t k = (size_t)kx * N + ky;
-        cplx vx = lds[kx * S2 + c], vy = lds[kx * S2 + W + c];
+        cplx vx = lds[lpos<S2, 1, 3>(kx, c)], vy = lds[lpos<S2, 1, 3>(kx, W + c)];
         double fx = 0.0, fy = 0.0;
         if (kx != 0 || ky != 0) {
             const double ph = q4[kx] + q4[ky];
@@ -199,40 +257,40 @@ __global__ __launch_bounds__(ColE<N>::NT) void k_col_kmult(cplx* __restrict__ Z,
             const double s = sa[k];
             vx = cscale(vx, s);
             vy = cscale(vy, s);
-            lds[kx * S2 + c] = cscale(vx, invV);
-            lds[kx * S2 + W + c] = cscale(vy, invV);
+            lds[lpos<S2, 1, 3>(kx, c)] = cscale(vx, invV);
+            lds[lpos<S2, 1, 3>(kx, W + c)] = cscale(vy, invV);
         }
         const cplx z = mk(-(fx * vx.y + fy * vy.y), fx * vx.x + fy * vy.x);   // i (fx vx + fy vy)
         if (CORR) zreg[q] = z;
-        else lds[kx * S2 + c] = z;
+        else lds[lpos<S2, 1, 3>(kx, c)] = z;
     }
     __syncthreads();
     if (CORR) {
         // xi~ = F_col^-1 of the scaled tiles, written back over X and Y
-        lds_fft<N, S2, S2, 1, NT, +1>(lds, tw, tid);
+        lds_fft<N, S2, S2, 1, NT, +1, 3>(lds, tw, tid);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = tid + q * NT;
             const int x = e / S2, s = e % S2;
             const size_t g = pofs + (size_t)x * N + col0 + (s % W);
-            if (s < W) X[g] = lds[e];
-            else Y[g] = lds[e];
+            if (s < W) X[g] = lds[lpos<S2, 1, 3>(x, s)];
+            else Y[g] = lds[lpos<S2, 1, 3>(x, s)];
         }
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PAIRS; q++) {
             const int pr = tid + q * NT;
-            lds[(pr / W) * S2 + (pr % W)] = zreg[q];
+            lds[lpos<S2, 1, 3>(pr / W, pr % W)] = zreg[q];
         }
         __syncthreads();
     }
     // Z: the W sequences in the X slots
-    lds_fft<N, W, S2, 1, NT, +1>(lds, tw, tid);
+    lds_fft<N, W, S2, 1, NT, +1, 3>(lds, tw, tid);
 #pragma unroll
     for (int q = 0; q < PAIRS; q++) {
         const int pr = tid + q * NT;
         const int x = pr / W, c = pr % W;
-        Z[pofs + (size_t)x * N + col0 + c] = lds[x * S2 + c];
+        Z[pofs + (size_t)x * N + col0 + c] = lds[lpos<S2, 1, 3>(x, c)];
     }
 }
 
@@ -246,21 +304,22 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_update(cplx* __restrict__ U
                                                            const cplx* __restrict__ ZB, size_t V, double ss,
                                                            const cplx* __restrict__ tw) {
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
-    __shared__ cplx lds[4 * N];
+    constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
+    __shared__ cplx lds[4 * PN];
     const int tid = threadIdx.x;
     const size_t row0 = (size_t)blockIdx.x * N;
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) lds[p * N + tid + q * NT] = ZA[p * V + row0 + tid + q * NT];
+        for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZA[p * V + row0 + tid + q * NT];
     __syncthreads();
-    lds_fft<N, 4, 1, N, NT, +1>(lds, tw, tid);
+    lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
     double ha[SPT][8];
 #pragma unroll
     for (int q = 0; q < SPT; q++)
 #pragma unroll
         for (int p = 0; p < 4; p++) {
-            const cplx a = lds[p * N + tid + q * NT];
+            const cplx a = lds[lpos<1, PN, 4>(tid + q * NT, p)];
             ha[q][2 * p] = a.x;
             ha[q][2 * p + 1] = a.y;
         }
@@ -268,9 +327,9 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_update(cplx* __restrict__ U
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) lds[p * N + tid + q * NT] = ZB[p * V + row0 + tid + q * NT];
+        for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZB[p * V + row0 + tid + q * NT];
     __syncthreads();
-    lds_fft<N, 4, 1, N, NT, +1>(lds, tw, tid);
+    lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
         const int y = tid + q * NT;
@@ -278,7 +337,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_update(cplx* __restrict__ U
         double hb[8];
 #pragma unroll
         for (int p = 0; p < 4; p++) {
-            const cplx b = lds[p * N + y];
+            const cplx b = lds[lpos<1, PN, 4>(y, p)];
             hb[2 * p] = b.x;
             hb[2 * p + 1] = b.y;
         }
@@ -317,11 +376,13 @@ static hipError_t step_fused_n(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA,
                                Xi + 4 * V, V, sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (corr && (e = fft_rows(c, Xi, 8, 0, 1.0)) != hipSuccess) return e;
     {
         KTimer t(c, kcls[2]);
-        hipLaunchKernelGGL((k_row_uxiu_fwd<N>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U,
-                           corr ? Xi : nullptr, V, ns, sa_pos, c->tw);
+        if (corr)
+            hipLaunchKernelGGL((k_row_uxiu_corr<N>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, Xi, V, c->tw);
+        else
+            hipLaunchKernelGGL((k_row_uxiu_fwd<N>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, nullptr, V, ns,
+                               sa_pos, c->tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     {

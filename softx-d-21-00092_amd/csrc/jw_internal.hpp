@@ -107,7 +107,8 @@ hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB
 hipError_t corr_from_Uk(jw_ctx* c, const cplx* Uk, double* corr);
 hipError_t accumulate_real(jw_ctx* c, const double* corr, double* sum, double* err);
 
-// unit-test hooks
+// instrumentation and unit-test hooks
+hipError_t calibration_copy(jw_ctx* c, const cplx* a, cplx* b, size_t n);
 hipError_t exp_batch(jw_ctx* c, const double* h8, double s, double* out18, int n);
 
 }  // namespace jw

@@ -43,26 +43,29 @@ template <int N, int DIR>
 __global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_rows(cplx* __restrict__ data, size_t plane_stride,
                                                              const cplx* __restrict__ tw, double scale) {
     using C = RowCfg<N>;
-    __shared__ cplx lds[C::ELEMS];
+    constexpr int PN = padded<N, 4>::value;            // padded row length in LDS
+    __shared__ cplx lds[C::RPW * PN];
     const int tid = threadIdx.x;
     cplx* base = data + blockIdx.y * plane_stride + (size_t)blockIdx.x * C::ELEMS;
-    for (int i = tid; i < C::ELEMS; i += C::NT) lds[i] = base[i];
+    for (int i = tid; i < C::ELEMS; i += C::NT) lds[lpos<1, PN, 4>(i % N, i / N)] = base[i];
     __syncthreads();
-    lds_fft<N, C::RPW, 1, N, C::NT, DIR>(lds, tw, tid);
-    for (int i = tid; i < C::ELEMS; i += C::NT) base[i] = cscale(lds[i], scale);
+    lds_fft<N, C::RPW, 1, PN, C::NT, DIR, 4>(lds, tw, tid);
+    for (int i = tid; i < C::ELEMS; i += C::NT) base[i] = cscale(lds[lpos<1, PN, 4>(i % N, i / N)], scale);
 }
 
 template <int N, int DIR>
 __global__ __launch_bounds__(ColCfg<N>::NT) void k_fft_cols(cplx* __restrict__ data, size_t plane_stride,
                                                              const cplx* __restrict__ tw, double scale) {
     using C = ColCfg<N>;
-    __shared__ cplx lds[C::ELEMS];
+    __shared__ cplx lds[padded<N, 3>::value * C::W];
     const int tid = threadIdx.x;
     cplx* base = data + blockIdx.y * plane_stride + (size_t)blockIdx.x * C::W;
-    for (int i = tid; i < C::ELEMS; i += C::NT) lds[i] = base[(size_t)(i / C::W) * N + (i % C::W)];
+    for (int i = tid; i < C::ELEMS; i += C::NT)
+        lds[lpos<C::W, 1, 3>(i / C::W, i % C::W)] = base[(size_t)(i / C::W) * N + (i % C::W)];
     __syncthreads();
-    lds_fft<N, C::W, C::W, 1, C::NT, DIR>(lds, tw, tid);
-    for (int i = tid; i < C::ELEMS; i += C::NT) base[(size_t)(i / C::W) * N + (i % C::W)] = cscale(lds[i], scale);
+    lds_fft<N, C::W, C::W, 1, C::NT, DIR, 3>(lds, tw, tid);
+    for (int i = tid; i < C::ELEMS; i += C::NT)
+        base[(size_t)(i / C::W) * N + (i % C::W)] = cscale(lds[lpos<C::W, 1, 3>(i / C::W, i % C::W)], scale);
 }
 
 template <int N>
@@ -598,6 +601,19 @@ __global__ __launch_bounds__(256) void k_mul_exp_packed(cplx* __restrict__ U, co
 hipError_t mul_exp_packed(jw_ctx* c, cplx* U, const cplx* R) {
     KTimer t(c, K_MV);
     hipLaunchKernelGGL(k_mul_exp_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, U, R, c->V);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// calibration: a plain copy of n double2 (16 n bytes read, 16 n written), used
+// to scale the HBM traffic counters of the other kernels
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_calibration_copy(const cplx* __restrict__ a, cplx* __restrict__ b, size_t n) {
+    GRID_STRIDE(i, n) b[i] = a[i];
+}
+
+hipError_t calibration_copy(jw_ctx* c, const cplx* a, cplx* b, size_t n) {
+    hipLaunchKernelGGL(k_calibration_copy, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, a, b, n);
     return hipGetLastError();
 }
 

@@ -81,6 +81,7 @@ def lib():
         L.jw_su3_exp_gen.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, ctypes.c_int]
         L.jw_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_set_fused.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_calibration_copy.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -226,6 +227,9 @@ class Context:
     # instrumentation
     def timing(self, enable=True):
         _check(lib().jw_timing(self.h, 1 if enable else 0))
+
+    def calibration_copy(self, nplanes=8):
+        _check(lib().jw_calibration_copy(self.h, nplanes))
 
     def set_fused(self, enable=True):
         _check(lib().jw_set_fused(self.h, 1 if enable else 0))
