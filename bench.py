@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--calibrate", action="store_true",
                    help="also run a plain copy of 8 planes (k_calibration_copy) to scale traffic counters")
     p.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    p.add_argument("--fused-mode", type=int, default=1,
+                   help="jw_set_fused mode: 1 default, 2/3/4 row-major/4-row/8-row blocked intermediates")
     return p.parse_args()
 
 
@@ -68,6 +70,7 @@ def main():
         al = None
 
     ctx = jw.Context(N, 0)
+    ctx.set_fused(a.fused_mode)
     U = ctx.field()
     ctx.philox(1, 0, 0)
     g2mu = 0.25
@@ -98,13 +101,14 @@ def main():
     # step (complex planes of 16 B each way; U is 9 complex planes):
     #   row_noise_fwd  W 8 planes                                  128
     #   col_kmult      A: R 8 W 4 (+W 8 correlated), B: R 8 W 4    384 (512)
-    #   fft_rows       correlated only: inverse rows of xi~ R 8 W 8  (256)
     #   row_uxiu_fwd   R U 9 (+ xi~ 8 correlated), W 8             272 (400)
     #   row_update     R A 4 + B 4 + U 9, W U 9                    416
+    # Correlated: 1456 B = 182 reals per site, the compulsory figure.  White
+    # noise: 1200 B = 150 reals (U is read by both row_uxiu_fwd and row_update).
     kind = "correlated" if a.coupling == "correlated" else "white"
     corr_ = kind == "correlated"
     alg = {"row_noise_fwd": 128, "col_kmult": 512 if corr_ else 384, "row_uxiu_fwd": 400 if corr_ else 272,
-           "fft_rows": 256, "row_update": 416}
+           "row_update": 416}
     V = N * N
     nk = min(a.steps, 10)
     per_step_ms = {k: ms * n / nk for k, (ms, n) in kt.items()}

@@ -163,9 +163,12 @@ int jw_radial_bins(int N, const double* S, int nbins, double* rbar, double* Sbin
 int jw_saturation_scale(int nbins, const double* rbar, const double* Sbin, const double* count, double* Qs);
 
 /* ---- instrumentation ------------------------------------------------------ */
-/* Select the fused row/column passes (default, 64 <= N <= 2048) or the unfused
- * kernel sequence for jw_evolve; both compute the same step. */
-int jw_set_fused(jw_ctx* ctx, int enable);
+/* Select how jw_evolve runs the generator-basis step; all modes compute the
+ * same step.  mode 0: the unfused kernel sequence; 1: the fused row/column
+ * passes (default, 64 <= N <= 2048) with the default layout of their
+ * intermediate planes; 2, 3, 4: the fused passes with row-major, 4-row-blocked
+ * or 8-row-blocked intermediate planes (for measurement). */
+int jw_set_fused(jw_ctx* ctx, int mode);
 /* Enable per-kernel timing with HIP events on the context's stream. */
 int jw_timing(jw_ctx* ctx, int enable);
 /* Average duration (ms) and call count of kernel class `which` (see jw_kernel_name). */

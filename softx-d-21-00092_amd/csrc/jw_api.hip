@@ -61,6 +61,17 @@ int ensure_sa(jw_ctx* c, int variant, const jw::Alpha& a) {
     HIP_CHECK(jw::build_sqrt_alpha(c, variant, a));
     c->sa_variant = variant;
     c->sa_params = a;
+    c->sa_gen += 1;
+    return JW_OK;
+}
+
+// the current sa table in 16-row blocks (for the blocked column pass)
+int ensure_sa16(jw_ctx* c) {
+    if (!c->sa16 && hipMalloc(&c->sa16, c->V * sizeof(double)) != hipSuccess)
+        return fail(JW_ERR_ALLOC, "coupling table allocation failed");
+    if (c->sa16_gen == c->sa_gen) return JW_OK;
+    HIP_CHECK(jw::block16(c, c->sa16, c->sa));
+    c->sa16_gen = c->sa_gen;
     return JW_OK;
 }
 
@@ -85,6 +96,40 @@ int check_field(const jw_field* f, const jw_ctx* c, int t, const char* name) {
     return JW_OK;
 }
 
+// the fused step's intermediate layout when none is selected: measured at
+// N = 2048, 4-row blocks for correlated noise, row-major for white noise
+int default_layout(int N, int corr) {
+    (void)N;
+    return corr ? 4 : 0;
+}
+
+// nsteps generator-basis steps: the fused passes or the unfused kernel sequence
+int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, jw::cplx* Xi, jw::cplx* Vv, jw::cplx* ZA,
+                           jw::cplx* ZB, const double* sa_pos, const double* sa_k, int kc, int corr, double s,
+                           double ss) {
+    const size_t V = c->V;
+    for (int it = 0; it < nsteps; it++) {
+        bool ok;
+        jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
+        if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
+        if (fused) {
+            HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc == 0, corr, ss));
+            continue;
+        }
+        HIP_CHECK(jw::gen_noise_packed(c, Xi, ns, sa_pos));
+        HIP_CHECK(jw::fft2d(c, Xi, 8, 1, s));
+        HIP_CHECK(jw::kernel_mult_packed(c, ZA, Xi, Xi + 4 * V, kc, sa_k, corr));
+        HIP_CHECK(jw::fft2d(c, ZA, 4, 0, s));
+        if (corr) HIP_CHECK(jw::fft2d(c, Xi, 8, 0, s));
+        HIP_CHECK(jw::uxiu_packed(c, Vv, U->d, corr ? Xi : nullptr, ns, sa_pos));
+        HIP_CHECK(jw::fft2d(c, Vv, 8, 1, s));
+        HIP_CHECK(jw::kernel_mult_packed(c, ZB, Vv, Vv + 4 * V, kc, sa_k, 0));
+        HIP_CHECK(jw::fft2d(c, ZB, 4, 0, s));
+        HIP_CHECK(jw::update_packed(c, U->d, ZA, ZB, ss));
+    }
+    return JW_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -106,6 +151,7 @@ int jw_create(jw_ctx** out, int N, int device) {
     c->V = (size_t)N * N;
     c->device = device;
     c->sa_variant = -1;
+    c->layout = JW_DEFAULT_LAYOUT;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(JW_ERR_HIP, "stream creation failed");
@@ -142,6 +188,7 @@ void jw_destroy(jw_ctx* c) {
     if (c->q4) hipFree(c->q4);
     if (c->sb) hipFree(c->sb);
     if (c->sa) hipFree(c->sa);
+    if (c->sa16) hipFree(c->sa16);
     if (c->inj) hipFree(c->inj);
     if (c->ws) hipFree(c->ws);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -430,26 +477,15 @@ int jw_evolve(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, con
     const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
     const double s = 1.0 / c->N, ss = sqrt(step);
     const bool fused = jw::fused_supported(c->N) && !c->unfused;
-    for (int it = 0; it < nsteps; it++) {
-        bool ok;
-        jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
-        if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
-        if (fused) {
-            HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc == 0, corr, ss));
-            continue;
-        }
-        HIP_CHECK(jw::gen_noise_packed(c, Xi, ns, sa_pos));
-        HIP_CHECK(jw::fft2d(c, Xi, 8, 1, s));
-        HIP_CHECK(jw::kernel_mult_packed(c, ZA, Xi, Xi + 4 * V, kc, sa_k, corr));
-        HIP_CHECK(jw::fft2d(c, ZA, 4, 0, s));
-        if (corr) HIP_CHECK(jw::fft2d(c, Xi, 8, 0, s));
-        HIP_CHECK(jw::uxiu_packed(c, Vv, U->d, corr ? Xi : nullptr, ns, sa_pos));
-        HIP_CHECK(jw::fft2d(c, Vv, 8, 1, s));
-        HIP_CHECK(jw::kernel_mult_packed(c, ZB, Vv, Vv + 4 * V, kc, sa_k, 0));
-        HIP_CHECK(jw::fft2d(c, ZB, 4, 0, s));
-        HIP_CHECK(jw::update_packed(c, U->d, ZA, ZB, ss));
+    const int saved_layout = c->layout;
+    if (c->layout < 0) c->layout = default_layout(c->N, corr);
+    if (fused && sa_k && c->layout != 0) {
+        if (int e = ensure_sa16(c)) { c->layout = saved_layout; return e; }
+        sa_k = c->sa16;
     }
-    return JW_OK;
+    const int rc = evolve_generator_basis(c, U, nsteps, fused, Xi, Vv, ZA, ZB, sa_pos, sa_k, kc, corr, s, ss);
+    c->layout = saved_layout;
+    return rc;
 }
 
 int jw_mv_init(jw_ctx* c, jw_field* U, int Ny, double g, double mu, double mass, const double* host) {
@@ -555,9 +591,12 @@ static const char* kNames[] = {"fft_rows", "fft_cols", "gen_noise", "kernel_mult
                                "update", "mv", "observables", "reference_functions",
                                "row_noise_fwd", "col_kmult", "row_uxiu_fwd", "row_update"};
 
-int jw_set_fused(jw_ctx* c, int enable) {
+int jw_set_fused(jw_ctx* c, int mode) {
     if (!c) return fail(JW_ERR_ARG, "null context");
-    c->unfused = enable ? 0 : 1;
+    static const int layouts[5] = {0, JW_DEFAULT_LAYOUT, 0, 4, 8};
+    if (mode < 0 || mode > 4) return fail(JW_ERR_ARG, "fused mode %d: expected 0..4", mode);
+    c->unfused = mode ? 0 : 1;
+    c->layout = layouts[mode];
     return JW_OK;
 }
 

@@ -12,15 +12,29 @@
 // (units: complex planes of V doubles pairs, 16 B per site each).
 //
 // Row passes: one workgroup per lattice row, four planes at a time in LDS.
-// Column passes: W adjacent columns of one plane per workgroup, workgroups
-// mapped so that strips sharing cache lines run on the same XCD.
+//
+// The intermediate planes (xi, V, A, B spectra) between a row pass and a
+// column pass are stored in one of two layouts (template parameter BF):
+//   BF = 0   row-major, site (x, y) at x N + y.  Column passes then take strips
+//            of W adjacent columns (X and Y tiles together in LDS), workgroups
+//            mapped so that strips sharing cache lines run on the same XCD.
+//   BF = 4, 8  blocks of BF rows: (x, y) at ((x / BF) N + y) BF + x mod BF, so
+//            that one 128-byte segment holds 8 / BF adjacent columns of BF
+//            consecutive rows.  A column pass then reads and writes whole
+//            segments (8 / BF columns per workgroup, one plane at a time, all
+//            four plane pairs in one workgroup); row passes touch BF-row
+//            segments whose rows are processed on one XCD at neighbouring times.
+// U always stays row-major.
 #include "jw_common.hpp"
 
 namespace jw {
 
-template <int N>
+// Row passes: N / DIV threads (64..1024), SPT sites per thread.  At N = 2048
+// the four padded rows in LDS (139 KB) allow one workgroup per CU; 512 threads
+// (DIV = 4) measured faster than 1024 for these passes.
+template <int N, int DIV = 4>
 struct RowE {
-    static constexpr int NT = (N / 4 < 64) ? 64 : ((N / 4 > 512) ? 512 : N / 4);
+    static constexpr int NT = (N / DIV < 64) ? 64 : ((N / DIV > 1024) ? 1024 : N / DIV);
     static constexpr int SPT = N / NT;                  // sites per thread
     static_assert(SPT >= 1 && SPT * NT == N, "row pass needs N a multiple of its thread count");
 };
@@ -43,10 +57,45 @@ __device__ __forceinline__ int xcd_strip(int b, int nstrips) {
     return (b % 8) * (nstrips / 8) + b / 8;
 }
 
+// offset of site (x, y) in an intermediate plane of layout BF
+template <int N, int BF>
+__device__ __forceinline__ size_t lay(int x, int y) {
+    if constexpr (BF == 0) return (size_t)x * N + y;
+    else return ((size_t)(x / BF) * N + y) * BF + (x % BF);
+}
+
+// lattice row of row-pass block b.  Blocks are dealt round-robin to the 8 XCDs;
+// with a blocked layout the BF rows sharing segments go to one XCD in
+// consecutive turns (a bijection on [0, N) for N a multiple of 8 BF).
+template <int N, int BF>
+__device__ __forceinline__ int row_of_block(int b) {
+    if constexpr (BF == 0) return b;
+    else {
+        static_assert(N % (8 * BF) == 0, "blocked layout needs N a multiple of 8 BF");
+        const int t = b >> 3, xcd = b & 7;
+        return (t / BF) * (8 * BF) + xcd * BF + (t % BF);
+    }
+}
+
+// the same pointer, opaque to the optimiser: loads through it are not merged
+// with earlier loads of the same addresses (which would keep every twiddle of
+// one FFT live in registers until the next)
+template <typename T>
+__device__ __forceinline__ const T* opaque(const T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+// likewise for a per-thread index: values derived from it are recomputed
+// rather than kept live from one phase of a kernel to the next
+__device__ __forceinline__ int opaque_v(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // row pass 1: noise -> xi row spectra (planes 0..3 xi_x, 4..7 xi_y)
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int BF>
 __global__ __launch_bounds__(RowE<N>::NT) void k_row_noise_fwd(cplx* __restrict__ Xi, size_t V, NoiseSrc ns,
                                                               const double* __restrict__ sa_pos,
                                                               const cplx* __restrict__ tw) {
@@ -54,7 +103,8 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_noise_fwd(cplx* __restrict_
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
     __shared__ cplx lds[4 * PN];
     const int tid = threadIdx.x;
-    const size_t row0 = (size_t)blockIdx.x * N;
+    const int x = row_of_block<N, BF>(blockIdx.x);
+    const size_t row0 = (size_t)x * N;
     double m[SPT][8];
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
@@ -72,7 +122,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_noise_fwd(cplx* __restrict_
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Xi[p * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+        for (int q = 0; q < SPT; q++) Xi[p * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < SPT; q++)
@@ -83,43 +133,34 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_noise_fwd(cplx* __restrict_
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Xi[(4 + p) * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+        for (int q = 0; q < SPT; q++) Xi[(4 + p) * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
 }
 
 // ---------------------------------------------------------------------------
-// row pass 2: V_i = U xi_i U^dagger -> V row spectra (planes 0..3 V_x, 4..7 V_y)
-// xi from the noise source, or from position-space planes Xi_pos (correlated)
+// row pass 2: V_i = U xi_i U^dagger -> V row spectra (planes 0..3 V_x, 4..7 V_y),
+// xi regenerated from the noise source (white noise)
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int BF>
 __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(cplx* __restrict__ Vv, const cplx* __restrict__ U,
-                                                             const cplx* __restrict__ Xi_pos, size_t V, NoiseSrc ns,
-                                                             const double* __restrict__ sa_pos,
+                                                             size_t V, NoiseSrc ns, const double* __restrict__ sa_pos,
                                                              const cplx* __restrict__ tw) {
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
     __shared__ cplx lds[4 * PN];
     const int tid = threadIdx.x;
-    const size_t row0 = (size_t)blockIdx.x * N;
+    const int x = row_of_block<N, BF>(blockIdx.x);
+    const size_t row0 = (size_t)x * N;
     double vy[SPT][8];
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
         const int y = tid + q * NT;
         const size_t i = row0 + y;
         double hx[8], hy[8];
-        if (Xi_pos) {
+        double z[16];
+        normals16(ns, i, z);
+        const double s = sa_pos ? sa_pos[i] : 1.0;
 #pragma unroll
-            for (int p = 0; p < 4; p++) {
-                const cplx a = Xi_pos[p * V + i], b = Xi_pos[(4 + p) * V + i];
-                hx[2 * p] = a.x; hx[2 * p + 1] = a.y;
-                hy[2 * p] = b.x; hy[2 * p + 1] = b.y;
-            }
-        } else {
-            double z[16];
-            normals16(ns, i, z);
-            const double s = sa_pos ? sa_pos[i] : 1.0;
-#pragma unroll
-            for (int k = 0; k < 8; k++) { hx[k] = z[2 * k] * s; hy[k] = z[2 * k + 1] * s; }
-        }
+        for (int k = 0; k < 8; k++) { hx[k] = z[2 * k] * s; hy[k] = z[2 * k + 1] * s; }
         m3 u;
 #pragma unroll
         for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
@@ -134,7 +175,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(cplx* __restrict__
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Vv[p * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+        for (int q = 0; q < SPT; q++) Vv[p * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < SPT; q++)
@@ -145,7 +186,8 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(cplx* __restrict__
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Vv[(4 + p) * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+        for (int q = 0; q < SPT; q++)
+            Vv[(4 + p) * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
 }
 
 // ---------------------------------------------------------------------------
@@ -155,7 +197,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(cplx* __restrict__
 // overwrite its own xi~ slots in LDS before the forward row FFT; U is read once
 // per batch (the second read is served by the caches).
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int BF>
 __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_corr(cplx* __restrict__ Vv, const cplx* __restrict__ U,
                                                               const cplx* __restrict__ Xs, size_t V,
                                                               const cplx* __restrict__ tw) {
@@ -163,14 +205,15 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_corr(cplx* __restrict_
     constexpr int PN = padded<N, 4>::value;
     __shared__ cplx lds[4 * PN];
     const int tid = threadIdx.x;
-    const size_t row0 = (size_t)blockIdx.x * N;
+    const int x = row_of_block<N, BF>(blockIdx.x);
+    const size_t row0 = (size_t)x * N;
 #pragma unroll 1
     for (int half = 0; half < 2; half++) {
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
             for (int q = 0; q < SPT; q++)
-                lds[lpos<1, PN, 4>(tid + q * NT, p)] = Xs[(4 * half + p) * V + row0 + tid + q * NT];
+                lds[lpos<1, PN, 4>(tid + q * NT, p)] = Xs[(4 * half + p) * V + lay<N, BF>(x, tid + q * NT)];
         __syncthreads();
         lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
         // one site at a time: keeps a single U (9 complex) live
@@ -199,7 +242,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_corr(cplx* __restrict_
         for (int p = 0; p < 4; p++)
 #pragma unroll
             for (int q = 0; q < SPT; q++)
-                Vv[(4 * half + p) * V + row0 + tid + q * NT] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+                Vv[(4 * half + p) * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
         __syncthreads();
     }
 }
@@ -295,23 +338,149 @@ __global__ __launch_bounds__(ColE<N>::NT) void k_col_kmult(cplx* __restrict__ Z,
 }
 
 // ---------------------------------------------------------------------------
+// column pass on the blocked layout: one workgroup per 128-byte column segment
+// (CPB = 8 / BF adjacent columns) and plane pair p, one plane at a time in LDS:
+//   X^ = F_col X[p]  ->  acc = i fx X^       (CORR: X^ *= sa, X[p] <- F_col^-1[X^ / V])
+//   Y^ = F_col Y[p]  ->  acc += i fy Y^      (CORR: likewise for Y[p])
+//   Z[p] = F_col^-1 acc
+// with fx, fy, sa as in k_col_kmult.  Element e of a loaded segment set is
+// x-block e / 8, slot e mod 8 = c BF + x mod BF: consecutive threads read
+// consecutive 16-byte words.  sa16 holds sa(k) in 16-row blocks.
+// ---------------------------------------------------------------------------
+template <int N, int BF>
+struct ColB {
+    static constexpr int CPB = 8 / BF;                 // columns per workgroup
+    static constexpr int ELEMS = CPB * N;
+    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 512) ? 512 : ELEMS / 8);
+    static constexpr int PER = ELEMS / NT;             // elements per thread
+    static_assert((BF == 4 || BF == 8) && PER * NT == ELEMS && N % 16 == 0, "blocked column tiling");
+};
+
+template <int N, int BF, int CORR>
+__global__ __launch_bounds__((ColB<N, BF>::NT)) void k_col_kmult_blk(cplx* __restrict__ Z, cplx* __restrict__ X,
+                                                                  cplx* __restrict__ Y, size_t V, int sqrt_coupling,
+                                                                  const double* __restrict__ q4,
+                                                                  const double* __restrict__ sb,
+                                                                  const double* __restrict__ sa16,
+                                                                  const cplx* __restrict__ tw) {
+    constexpr int CPB = ColB<N, BF>::CPB, NT = ColB<N, BF>::NT, PER = ColB<N, BF>::PER;
+    constexpr int PN = padded<N, 4>::value;
+    __shared__ cplx lds[CPB * PN];
+    const int tid = threadIdx.x;
+    const int y0 = blockIdx.x * CPB;
+    const double invV = 1.0 / ((double)N * (double)N);
+    // global offset within a plane / LDS slot of loaded element e
+    auto goff = [&](int e) -> size_t { return (size_t)(e >> 3) * ((size_t)N * BF) + (size_t)y0 * BF + (e & 7); };
+    auto lofs = [&](int e) -> int {
+        const int sl = e & 7;
+        return lpos<1, PN, 4>((e >> 3) * BF + sl % BF, sl / BF);
+    };
+    auto load = [&](const cplx* __restrict__ P) {
+        const int t = opaque_v(tid);
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = t + q * NT;
+            lds[lofs(e)] = P[goff(e)];
+        }
+    };
+    auto store = [&](cplx* __restrict__ P) {
+        const int t = opaque_v(tid);
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = t + q * NT;
+            P[goff(e)] = lds[lofs(e)];
+        }
+    };
+    // one direction d (0: x, 1: y): forward column FFT of P, accumulate
+    // i f_d P^ into acc for this thread's pairs (kx = pr mod N, column c = pr / N)
+    // with f_d = -2 pi cc sb[k_d] / phat^2 / V, zero at k = 0; CORR: scale P^
+    // by sa and write F_col^-1[sa P^ / V] back over P
+    auto direction = [&](cplx* __restrict__ P, cplx acc[PER], int d) {
+        const cplx* twd = opaque(tw);
+        const double* q4d = opaque(q4);
+        const double* sbd = opaque(sb);
+        const double* sad = opaque(sa16);
+        load(P);
+        __syncthreads();
+        lds_fft<N, CPB, 1, PN, NT, -1, 4>(lds, twd, opaque_v(tid));
+        const int t = opaque_v(tid);
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int pr = t + q * NT;
+            const int kx = pr % N, c = pr / N, ky = y0 + c;
+            const int l = lpos<1, PN, 4>(kx, c);
+            const size_t k16 = ((size_t)(kx >> 4) * N + ky) * 16 + (kx & 15);
+            cplx v = lds[l];
+            if (CORR) {
+                v = cscale(v, sad[k16]);
+                lds[l] = cscale(v, invV);
+            }
+            double f = 0.0;
+            if (kx != 0 || ky != 0) {
+                const double ph = q4d[kx] + q4d[ky];
+                const double cc = sqrt_coupling ? sad[k16] : 1.0;
+                f = -2.0 * M_PI * cc * sbd[d ? ky : kx] / ph * invV;
+            }
+            if (d == 0) acc[q] = mk(-(f * v.y), f * v.x);
+            else acc[q] = mk(acc[q].x - f * v.y, acc[q].y + f * v.x);
+        }
+        __syncthreads();
+        if (CORR) {
+            lds_fft<N, CPB, 1, PN, NT, +1, 4>(lds, opaque(tw), opaque_v(tid));
+            store(P);
+            __syncthreads();
+        }
+    };
+    // plane pair p = blockIdx.y (one pass over the pairs would let the compiler
+    // hoist every twiddle and kernel factor out of the loop into registers)
+    const size_t po = (size_t)blockIdx.y * V;
+    cplx acc[PER];
+    direction(X + po, acc, 0);
+    direction(Y + po, acc, 1);
+    const int t = opaque_v(tid);
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int pr = t + q * NT;
+        lds[lpos<1, PN, 4>(pr % N, pr / N)] = acc[q];
+    }
+    __syncthreads();
+    lds_fft<N, CPB, 1, PN, NT, +1, 4>(lds, opaque(tw), opaque_v(tid));
+    store(Z + po);
+}
+
+__global__ __launch_bounds__(256) void k_block16(double* __restrict__ out, const double* __restrict__ in, int N) {
+    const size_t V = (size_t)N * N;
+    GRID_STRIDE(i, V) {
+        const int kx = (int)(i / N), ky = (int)(i % N);
+        out[((size_t)(kx >> 4) * N + ky) * 16 + (kx & 15)] = in[i];
+    }
+}
+
+hipError_t block16(jw_ctx* c, double* out, const double* in) {
+    if (c->N % 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_block16, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, out, in, c->N);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // row pass 3: inverse row FFTs of the A and B column-inverse spectra and the
 // update U <- exp(i s B) U exp(-i s A) (field.h:1966-1996) in one pass.
 // A's coefficients are kept in registers, B's stay in LDS.
 // ---------------------------------------------------------------------------
-template <int N>
-__global__ __launch_bounds__(RowE<N>::NT) void k_row_update(cplx* __restrict__ U, const cplx* __restrict__ ZA,
+template <int N, int BF>
+__global__ __launch_bounds__((RowE<N, 4>::NT)) void k_row_update(cplx* __restrict__ U, const cplx* __restrict__ ZA,
                                                            const cplx* __restrict__ ZB, size_t V, double ss,
                                                            const cplx* __restrict__ tw) {
-    constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
+    constexpr int NT = RowE<N, 4>::NT, SPT = RowE<N, 4>::SPT;
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
     __shared__ cplx lds[4 * PN];
     const int tid = threadIdx.x;
-    const size_t row0 = (size_t)blockIdx.x * N;
+    const int x = row_of_block<N, BF>(blockIdx.x);
+    const size_t row0 = (size_t)x * N;
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZA[p * V + row0 + tid + q * NT];
+        for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZA[p * V + lay<N, BF>(x, tid + q * NT)];
     __syncthreads();
     lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
     double ha[SPT][8];
@@ -327,7 +496,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_update(cplx* __restrict__ U
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZB[p * V + row0 + tid + q * NT];
+        for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZB[p * V + lay<N, BF>(x, tid + q * NT)];
     __syncthreads();
     lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
 #pragma unroll
@@ -354,46 +523,62 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_update(cplx* __restrict__ U
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int BF>
+static hipError_t col_pass(jw_ctx* c, cplx* Z, cplx* X, cplx* Y, const double* sa_k, int sqrt_coupling, int corr) {
+    const size_t V = c->V;
+    if constexpr (BF == 0) {
+        constexpr int W = ColE<N>::W;
+        if (corr)
+            hipLaunchKernelGGL((k_col_kmult<N, 1>), dim3(N / W, 4), dim3(ColE<N>::NT), 0, c->stream, Z, X, Y, V,
+                               sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
+        else
+            hipLaunchKernelGGL((k_col_kmult<N, 0>), dim3(N / W, 4), dim3(ColE<N>::NT), 0, c->stream, Z, X, Y, V,
+                               sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
+    } else {
+        constexpr int CPB = ColB<N, BF>::CPB, NT = ColB<N, BF>::NT;
+        if (corr)
+            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1>), dim3(N / CPB, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
+                               sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
+        else
+            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 0>), dim3(N / CPB, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
+                               sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
+    }
+    return hipGetLastError();
+}
+
+template <int N, int BF>
 static hipError_t step_fused_n(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
                                const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr,
-                               double sqrt_step, int* kcls) {
+                               double sqrt_step, const int* kcls) {
     const size_t V = c->V;
-    constexpr int W = ColE<N>::W;
     hipError_t e;
     {
         KTimer t(c, kcls[0]);
-        hipLaunchKernelGGL((k_row_noise_fwd<N>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Xi, V, ns, sa_pos, c->tw);
+        hipLaunchKernelGGL((k_row_noise_fwd<N, BF>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Xi, V, ns, sa_pos,
+                           c->tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[1]);
-        if (corr)
-            hipLaunchKernelGGL((k_col_kmult<N, 1>), dim3(N / W, 4), dim3(ColE<N>::NT), 0, c->stream, ZA, Xi,
-                               Xi + 4 * V, V, sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
-        else
-            hipLaunchKernelGGL((k_col_kmult<N, 0>), dim3(N / W, 4), dim3(ColE<N>::NT), 0, c->stream, ZA, Xi,
-                               Xi + 4 * V, V, sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = col_pass<N, BF>(c, ZA, Xi, Xi + 4 * V, sa_k, sqrt_coupling, corr)) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[2]);
         if (corr)
-            hipLaunchKernelGGL((k_row_uxiu_corr<N>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, Xi, V, c->tw);
+            hipLaunchKernelGGL((k_row_uxiu_corr<N, BF>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, Xi, V,
+                               c->tw);
         else
-            hipLaunchKernelGGL((k_row_uxiu_fwd<N>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, nullptr, V, ns,
+            hipLaunchKernelGGL((k_row_uxiu_fwd<N, BF>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, V, ns,
                                sa_pos, c->tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[1]);
-        hipLaunchKernelGGL((k_col_kmult<N, 0>), dim3(N / W, 4), dim3(ColE<N>::NT), 0, c->stream, ZB, Vv, Vv + 4 * V,
-                           V, sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = col_pass<N, BF>(c, ZB, Vv, Vv + 4 * V, sa_k, sqrt_coupling, 0)) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[3]);
-        hipLaunchKernelGGL((k_row_update<N>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, U, ZA, ZB, V, sqrt_step,
+        hipLaunchKernelGGL((k_row_update<N, BF>), dim3(N), dim3(RowE<N, 4>::NT), 0, c->stream, U, ZA, ZB, V, sqrt_step,
                            c->tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
@@ -402,18 +587,33 @@ static hipError_t step_fused_n(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA,
 
 bool fused_supported(int N) { return N >= 64 && N <= 2048; }
 
-hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
-                      const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr, double sqrt_step) {
-    int kcls[4] = {K_ROW_NOISE, K_COL_KMULT, K_ROW_UXIU, K_ROW_UPDATE};
-    switch (c->N) {
-        case 64: return step_fused_n<64>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
-        case 128: return step_fused_n<128>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
-        case 256: return step_fused_n<256>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
-        case 512: return step_fused_n<512>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
-        case 1024: return step_fused_n<1024>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
-        case 2048: return step_fused_n<2048>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
+template <int N>
+static hipError_t step_fused_layout(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
+                                    const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr,
+                                    double sqrt_step, const int* kcls) {
+    switch (c->layout) {
+        case 0: return step_fused_n<N, 0>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
+        case 4: return step_fused_n<N, 4>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
+        case 8: return step_fused_n<N, 8>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
+                      const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr, double sqrt_step) {
+    static const int kcls[4] = {K_ROW_NOISE, K_COL_KMULT, K_ROW_UXIU, K_ROW_UPDATE};
+#define JW_FUSED_CASE(n) \
+    case n: return step_fused_layout<n>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
+    switch (c->N) {
+        JW_FUSED_CASE(64)
+        JW_FUSED_CASE(128)
+        JW_FUSED_CASE(256)
+        JW_FUSED_CASE(512)
+        JW_FUSED_CASE(1024)
+        JW_FUSED_CASE(2048)
+        default: return hipErrorInvalidValue;
+    }
+#undef JW_FUSED_CASE
 }
 
 }  // namespace jw

@@ -8,6 +8,10 @@
 
 #include "../../include/jimwlk.h"
 
+// intermediate-plane layout of the fused step unless jw_set_fused selects one:
+// -1 picks per step kind (jw_default_layout in jw_api.hip)
+#define JW_DEFAULT_LAYOUT (-1)
+
 // Philox counter tags of the two noise streams
 #define JW_TAG_LANGEVIN 0x4C414E47u   // 'LANG'
 #define JW_TAG_MV 0x4D565348u         // 'MVSH'
@@ -47,6 +51,11 @@ struct jw_ctx {
     double* sa;
     int sa_variant;        // -1 none, 0 kernel formula, 1 noise formula
     jw::Alpha sa_params;
+    unsigned sa_gen;       // bumped whenever sa is rebuilt
+    // k-space sqrt(alpha) in 16-row blocks for the blocked column pass:
+    // sa16[((kx / 16) N + ky) 16 + kx mod 16] = sa[kx N + ky]
+    double* sa16;
+    unsigned sa16_gen;     // sa_gen it was built from (0: never)
     // injected normals (AoS, up to 16 per site), valid for one generator call
     double* inj;
     int inj_per_site;      // 0 when no injection is pending
@@ -57,6 +66,9 @@ struct jw_ctx {
     size_t ws_planes;
     // 1: evolve through the unfused kernel sequence even where the fused passes exist
     int unfused;
+    // fused step's intermediate-plane layout: 0 row-major, 4 or 8 rows per block,
+    // -1 default per step kind
+    int layout;
     // per-kernel timing (HIP events on ctx->stream)
     int timing;
     double kernel_ms[16];
@@ -100,8 +112,10 @@ hipError_t mul_exp_packed(jw_ctx* c, cplx* U, const cplx* R);
 
 // fused generator-basis step (64 <= N <= 2048)
 bool fused_supported(int N);
+// sa_k: row-major for layout 0, the 16-row-blocked sa16 otherwise
 hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
                       const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr, double sqrt_step);
+hipError_t block16(jw_ctx* c, double* out, const double* in);
 
 // observables
 hipError_t corr_from_Uk(jw_ctx* c, const cplx* Uk, double* corr);

@@ -231,8 +231,13 @@ class Context:
     def calibration_copy(self, nplanes=8):
         _check(lib().jw_calibration_copy(self.h, nplanes))
 
-    def set_fused(self, enable=True):
-        _check(lib().jw_set_fused(self.h, 1 if enable else 0))
+    # jw_set_fused modes
+    UNFUSED, FUSED, FUSED_ROWMAJOR, FUSED_BLOCK4, FUSED_BLOCK8 = range(5)
+
+    def set_fused(self, mode=1):
+        """mode: False/0 unfused kernel sequence, True/1 fused (default layout),
+        2/3/4 fused with row-major / 4-row / 8-row blocked intermediate planes."""
+        _check(lib().jw_set_fused(self.h, int(mode)))
 
     def kernel_times(self):
         out = {}

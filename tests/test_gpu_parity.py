@@ -261,21 +261,25 @@ def test_evolve_one_step(jw, orc, ctx64, coupling, kernel):
 @pytest.mark.parametrize("N", [64, 128, 256, 512, 1024, 2048])
 @pytest.mark.parametrize("coupling", [3, 1, 0, 4])
 def test_fused_matches_unfused(jw, orc, N, coupling):
-    """The fused row/column passes against the unfused kernel sequence, Philox noise."""
+    """The fused row/column passes, with each layout of their intermediate planes
+    (row-major, 4-row and 8-row blocks), against the unfused kernel sequence;
+    Philox noise, two steps."""
     c = jw.Context(N)
     U0 = c.field()
     c.philox(5, 1, 0)
     c.mv_init(U0, 2, 1.0, 0.25, 1.0)
     ref = U0.download()
-    results = []
-    for fused in (0, 1):
-        jw.lib().jw_set_fused(c.h, fused)
+    al = jw.alpha(lambda_L=0.02 * N, mu0_over_lambda=3.0) if coupling in (0, 4) else None
+    results = {}
+    for mode in (c.UNFUSED, c.FUSED_ROWMAJOR, c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED):
+        c.set_fused(mode)
         U = c.field(9, ref)
         c.philox(5, 1, 100)
-        c.evolve(U, coupling, 1, 1e-4, nsteps=2)
-        results.append(U.download())
-    assert frob_max(results[0], results[1]) < 1e-13
-    assert frob_max(results[0], ref) > 1e-3
+        c.evolve(U, coupling, 1, 1e-4, al, nsteps=2)
+        results[mode] = U.download()
+    for mode in (c.FUSED_ROWMAJOR, c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED):
+        assert frob_max(results[c.UNFUSED], results[mode]) < 1e-13, mode
+    assert frob_max(results[c.UNFUSED], ref) > 1e-3
     c.close()
 
 
@@ -299,6 +303,30 @@ def test_evolve_multi_step_injected(jw, orc, ctx64):
         ctx64.evolve(Ug, 3, 1, 1e-4, nsteps=1)
         Uo = orc.langevin_step(Uo, z, N, 3, 1, 1e-4)
     assert frob_max(Ug.download(), Uo) <= 1e-10
+
+
+@pytest.mark.parametrize("coupling", [3, 4])
+def test_evolve_1000_steps(jw, orc, ctx64, coupling):
+    """1000 steps from an MV-like start, FIXED and per-mode correlated noise
+    (alpha frozen at 0.7, Lambda a = 0.02), Philox noise drawn on the device
+    and by the oracle's restatement of the stream: per-site Frobenius <= 1e-9."""
+    import math
+    N, steps, ds = 64, 1000, 1e-4 if coupling == 3 else 5e-5
+    al_g = jw.alpha(lambda_L=0.02 * N, mu0_over_lambda=math.exp(2 * math.pi / (9 * 0.7))) if coupling == 4 else None
+    al_o = None
+    if coupling == 4:
+        import oracle
+        al_o = oracle.alpha(lambda_L=0.02 * N, mu0_over_lambda=math.exp(2 * math.pi / (9 * 0.7)))
+    U0 = random_su3(orc, N * N, scale=1.0)
+    U = ctx64.field(9, U0)
+    ctx64.philox(11, 2, 0)
+    ctx64.evolve(U, coupling, 1, ds, al_g, nsteps=steps)
+    Uo = U0
+    for s in range(steps):
+        Uo = orc.langevin_step(Uo, orc.philox_normals(N, 11, 2, s, jw.TAG_LANGEVIN, 16), N, coupling, 1, ds, al_o)
+    err = frob_max(U.download(), Uo)
+    assert err <= 1e-9, err
+    assert frob_max(Uo, U0) > 1e-2
 
 
 def test_evolve_philox_matches_oracle(jw, orc, ctx64):
