@@ -46,6 +46,9 @@ def parse():
     p.add_argument("--calibrate", action="store_true",
                    help="also run a plain copy of 8 planes (k_calibration_copy) to scale traffic counters")
     p.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    p.add_argument("--form", default="fp64", choices=["fp64", "mixed"],
+                   help="plain FP64 form, or the mixed-precision form (FP32 fields and FFTs)")
+    p.add_argument("--reunit-every", type=int, default=10)
     p.add_argument("--fused-mode", type=int, default=1,
                    help="jw_set_fused mode: 1 default, 2/3/4 row-major/4-row/8-row blocked intermediates")
     return p.parse_args()
@@ -83,10 +86,16 @@ def main():
     if a.calibrate:
         ctx.calibration_copy(8)
         ctx.sync()
-    ctx.evolve(U, coupling, jw.SIN_KERNEL, step, al, nsteps=a.warmup)
+    def run(n):
+        if a.form == "mixed":
+            ctx.evolve_mixed(U, coupling, jw.SIN_KERNEL, step, al, nsteps=n, reunit_every=a.reunit_every)
+        else:
+            ctx.evolve(U, coupling, jw.SIN_KERNEL, step, al, nsteps=n)
+
+    run(a.warmup)
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.evolve(U, coupling, jw.SIN_KERNEL, step, al, nsteps=a.steps)
+    run(a.steps)
     ctx.sync()
     dt = time.perf_counter() - t0
     sps = a.steps / dt
@@ -94,7 +103,7 @@ def main():
 
     # per-kernel durations with HIP events on the library stream (separate pass)
     ctx.timing(True)
-    ctx.evolve(U, coupling, jw.SIN_KERNEL, step, al, nsteps=min(a.steps, 10))
+    run(min(a.steps, 10))
     kt = ctx.kernel_times()
     ctx.timing(False)
     # Algorithmic HBM bytes per site per step of each kernel class of the fused

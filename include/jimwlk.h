@@ -141,6 +141,17 @@ int jw_reduce(jw_ctx* ctx, const jw_field* corr, jw_field* sum, jw_field* err);
 int jw_evolve(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
               int nsteps);
 
+/* The same evolution in the mixed-precision form (no counterpart in the
+ * reference, which is FP64 throughout; it replaces the same loop,
+ * main_optimized.cpp:234-311): U and every field and FFT in FP32, the SU(3)
+ * exponentials, products and the projection of U onto SU(3) in FP64.  U is
+ * converted to FP32 on entry and back on return, and projected onto SU(3)
+ * every reunit_every steps and after the last step (reunit_every = 0: never).
+ * The noise is the same Philox stream (normals computed in FP32 from the same
+ * words) or the pending injection.  SIN kernel, 64 <= N <= 2048. */
+int jw_evolve_mixed(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
+                    int nsteps, int reunit_every);
+
 /* MV initial condition (main_optimized.cpp:178-191): U = 1, then Ny sheets
  * U <- U exp(i A_s), A_s = F^-1[ F rho_s * (-g / (-phat2 + m^2)) ], rho_s with
  * sigma = g mu / sqrt(Ny), m = mass * g^2 * mu.  Charges: injected (8*V*Ny

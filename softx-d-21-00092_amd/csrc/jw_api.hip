@@ -96,10 +96,12 @@ int check_field(const jw_field* f, const jw_ctx* c, int t, const char* name) {
     return JW_OK;
 }
 
-// the fused step's intermediate layout when none is selected: measured at
-// N = 2048, 4-row blocks for correlated noise, row-major for white noise
-int default_layout(int N, int corr) {
+// the fused step's intermediate layout when none is selected, as measured at
+// N = 2048: FP64 storage, 4-row blocks for correlated noise and row-major for
+// white noise; FP32 storage, row-major
+int default_layout(int N, int corr, int fp32) {
     (void)N;
+    if (fp32) return 0;
     return corr ? 4 : 0;
 }
 
@@ -113,7 +115,8 @@ int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, jw::c
         jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
         if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
         if (fused) {
-            HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc == 0, corr, ss));
+            const jw::StepArgs sa{ns, sa_pos, sa_k, kc == 0, corr, ss, 0};
+            HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, sa));
             continue;
         }
         HIP_CHECK(jw::gen_noise_packed(c, Xi, ns, sa_pos));
@@ -156,11 +159,13 @@ int jw_create(jw_ctx** out, int N, int device) {
         delete c;
         return fail(JW_ERR_HIP, "stream creation failed");
     }
-    // twiddles exp(-2 pi i m/N) rounded from long double
+    // twiddles exp(-2 pi i m/N) rounded from long double (FP64 and FP32)
     std::vector<cplx> tw(N);
+    std::vector<float2> twf(N);
     for (int m = 0; m < N; m++) {
         long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)N;
         tw[m] = make_double2((double)cosl(ang), (double)sinl(ang));
+        twf[m] = make_float2((float)cosl(ang), (float)sinl(ang));
     }
     // momenta.h:90-125 per axis: 4 pow(sin(pi k/N), 2) and sin(2 pi k/N)
     std::vector<double> q4(N), sb(N);
@@ -169,11 +174,12 @@ int jw_create(jw_ctx** out, int N, int device) {
         sb[k] = sin(2.0 * M_PI * k / (1.0 * N));
     }
     if (hipMalloc(&c->tw, N * sizeof(cplx)) != hipSuccess || hipMalloc(&c->q4, N * sizeof(double)) != hipSuccess ||
-        hipMalloc(&c->sb, N * sizeof(double)) != hipSuccess) {
+        hipMalloc(&c->sb, N * sizeof(double)) != hipSuccess || hipMalloc(&c->twf, N * sizeof(float2)) != hipSuccess) {
         jw_destroy(c);
         return fail(JW_ERR_ALLOC, "table allocation failed");
     }
     hipMemcpy(c->tw, tw.data(), N * sizeof(cplx), hipMemcpyHostToDevice);
+    hipMemcpy(c->twf, twf.data(), N * sizeof(float2), hipMemcpyHostToDevice);
     hipMemcpy(c->q4, q4.data(), N * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(c->sb, sb.data(), N * sizeof(double), hipMemcpyHostToDevice);
     *out = c;
@@ -185,6 +191,7 @@ void jw_destroy(jw_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->tw) hipFree(c->tw);
+    if (c->twf) hipFree(c->twf);
     if (c->q4) hipFree(c->q4);
     if (c->sb) hipFree(c->sb);
     if (c->sa) hipFree(c->sa);
@@ -478,12 +485,69 @@ int jw_evolve(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, con
     const double s = 1.0 / c->N, ss = sqrt(step);
     const bool fused = jw::fused_supported(c->N) && !c->unfused;
     const int saved_layout = c->layout;
-    if (c->layout < 0) c->layout = default_layout(c->N, corr);
+    if (c->layout < 0) c->layout = default_layout(c->N, corr, 0);
     if (fused && sa_k && c->layout != 0) {
         if (int e = ensure_sa16(c)) { c->layout = saved_layout; return e; }
         sa_k = c->sa16;
     }
     const int rc = evolve_generator_basis(c, U, nsteps, fused, Xi, Vv, ZA, ZB, sa_pos, sa_k, kc, corr, s, ss);
+    c->layout = saved_layout;
+    return rc;
+}
+
+int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, const jw_alpha* a, int nsteps,
+                    int reunit_every) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (coupling == JW_HATTA_COUPLING_CONSTANT)
+        return fail(JW_ERR_UNSUPPORTED, "HATTA coupling: the reference leaves the coupling uninitialised here");
+    if (coupling < 0 || coupling > 4 || (kernel != 0 && kernel != 1)) return fail(JW_ERR_ARG, "bad coupling/kernel");
+    if (kernel != JW_SIN_KERNEL) return fail(JW_ERR_UNSUPPORTED, "the mixed-precision form runs the SIN kernel");
+    if (!jw::fused_supported(c->N)) return fail(JW_ERR_UNSUPPORTED, "the mixed-precision form needs 64 <= N <= 2048");
+    if (!(step >= 0)) return fail(JW_ERR_ARG, "step must be >= 0");
+    if (nsteps < 0 || reunit_every < 0) return fail(JW_ERR_ARG, "nsteps or reunit_every < 0");
+    if (c->inj_per_site && nsteps > 1) return fail(JW_ERR_STATE, "an injection covers one step; nsteps = %d", nsteps);
+    if (nsteps == 0) return JW_OK;
+    // FP32 state: U (9 planes) and the 24 intermediate planes, inside the
+    // workspace of 24 FP64 planes (48 FP32 planes)
+    if (int e = ensure_ws(c, 24)) return e;
+    const size_t V = c->V;
+    jw::cplxf* U32 = reinterpret_cast<jw::cplxf*>(c->ws);
+    jw::cplxf* Xi = U32 + 9 * V;
+    jw::cplxf* Vv = Xi + 8 * V;
+    jw::cplxf* ZA = Vv + 8 * V;
+    jw::cplxf* ZB = ZA + 4 * V;
+    const double* sa_pos = nullptr;
+    const double* sa_k = nullptr;
+    if (coupling == JW_NOISE_COUPLING_CONSTANT) {
+        if (int e = ensure_sa(c, 1, to_alpha(a))) return e;
+        sa_pos = c->sa;
+    } else if (coupling == JW_SQRT_COUPLING_CONSTANT || coupling == JW_CORRELATED_NOISE_COUPLING) {
+        if (int e = ensure_sa(c, 0, to_alpha(a))) return e;
+        sa_k = c->sa;
+    }
+    const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
+    const int saved_layout = c->layout;
+    if (c->layout < 0) c->layout = default_layout(c->N, corr, 1);
+    int rc = JW_OK;
+    if (sa_k && c->layout != 0) {
+        rc = ensure_sa16(c);
+        sa_k = c->sa16;
+    }
+    if (rc == JW_OK && jw::u_convert(c, U32, U->d, 1) != hipSuccess) rc = fail(JW_ERR_HIP, "U conversion failed");
+    for (int it = 0; rc == JW_OK && it < nsteps; it++) {
+        bool ok;
+        jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
+        if (!ok) {
+            rc = fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
+            break;
+        }
+        // projection every reunit_every steps and after the last one
+        const int reunit = reunit_every > 0 && ((it + 1) % reunit_every == 0 || it == nsteps - 1);
+        const jw::StepArgs sa{ns, sa_pos, sa_k, coupling == JW_SQRT_COUPLING_CONSTANT, corr, sqrt(step), reunit};
+        const hipError_t he = jw::step_fused_f32(c, U32, Xi, Vv, ZA, ZB, sa);
+        if (he != hipSuccess) rc = fail(JW_ERR_HIP, "%s", hipGetErrorString(he));
+    }
+    if (rc == JW_OK && jw::u_convert(c, U32, U->d, 0) != hipSuccess) rc = fail(JW_ERR_HIP, "U conversion failed");
     c->layout = saved_layout;
     return rc;
 }

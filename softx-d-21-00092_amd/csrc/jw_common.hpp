@@ -99,6 +99,28 @@ __device__ __forceinline__ void normals8(const NoiseSrc& ns, size_t site, double
     }
 }
 
+// 16 normals of a site in the storage precision: FP64 pairs, or the FP32 pairs
+// of the same Philox words (box_muller_pair_f); injected normals are rounded
+template <typename T>
+__device__ __forceinline__ void normals16_t(const NoiseSrc& ns, size_t site, T z[16]) {
+    if constexpr (sizeof(T) == sizeof(double)) {
+        normals16(ns, site, z);
+    } else if (ns.inj) {
+        const double2* p = reinterpret_cast<const double2*>(ns.inj + 16 * site);
+#pragma unroll
+        for (int q = 0; q < 8; q++) { const double2 v = p[q]; z[2 * q] = (T)v.x; z[2 * q + 1] = (T)v.y; }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            box_muller_pair_f((uint32_t)site, ns.step, ns.tag, (uint32_t)q, ns.seed, ns.traj, z[2 * q], z[2 * q + 1]);
+    }
+}
+
+// scalar type of a complex storage type
+template <typename C> struct scalar_of;
+template <> struct scalar_of<double2> { typedef double type; };
+template <> struct scalar_of<float2> { typedef float type; };
+
 // the reference's lambda-basis site map (field.h:2169-2231; setMVModel field.h:714-745),
 // operation for operation
 __device__ __forceinline__ void lambda_entries(const double n[8], cplx u[9]) {

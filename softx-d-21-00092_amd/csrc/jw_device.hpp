@@ -25,6 +25,29 @@ __device__ __forceinline__ cplx cmuli(cplx a) { return mk(-a.y, a.x); }       //
 __device__ __forceinline__ cplx cmulmi(cplx a) { return mk(a.y, -a.x); }      // -i*a
 
 // ---------------------------------------------------------------------------
+// complex float (storage and FFTs of the mixed-precision form): the same
+// operations, overloaded on float2
+// ---------------------------------------------------------------------------
+typedef float2 cplxf;
+
+__device__ __forceinline__ cplxf mkf(float r, float i) { return make_float2(r, i); }
+__device__ __forceinline__ cplxf cadd(cplxf a, cplxf b) { return mkf(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cplxf csub(cplxf a, cplxf b) { return mkf(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cplxf cmul(cplxf a, cplxf b) { return mkf(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cplxf cscale(cplxf a, float s) { return mkf(a.x * s, a.y * s); }
+__device__ __forceinline__ cplxf cmuli(cplxf a) { return mkf(-a.y, a.x); }
+__device__ __forceinline__ cplxf cmulmi(cplxf a) { return mkf(a.y, -a.x); }
+
+// complex value of type C (double2 or float2) from two doubles, and back
+template <typename C>
+__device__ __forceinline__ C mkc(double r, double i) {
+    if constexpr (sizeof(C) == sizeof(double2)) return make_double2(r, i);
+    else return make_float2((float)r, (float)i);
+}
+__device__ __forceinline__ cplx to_d(cplx a) { return a; }
+__device__ __forceinline__ cplx to_d(cplxf a) { return make_double2((double)a.x, (double)a.y); }
+
+// ---------------------------------------------------------------------------
 // 3x3 complex matrices, row-major (entry k = 3*row + col, as field.h:199-209)
 // ---------------------------------------------------------------------------
 struct m3 { cplx m[9]; };
@@ -173,38 +196,94 @@ __device__ __forceinline__ void box_muller_pair(uint32_t site, uint32_t step, ui
     z1 = rr * s;
 }
 
+// The same pair in FP32 from the same Philox words (mixed-precision form):
+// log u0 from the 53-bit integer (through log1p of 1 - u0 above 1/2, so that
+// small |z| keep their relative precision) and the angle as pi x with
+// x = 2 u1 - 1 in (-1, 1): sincospi(x + 1) = -sincospi(x).  Each normal agrees
+// with the FP64 pair to a few FP32 ulps of max(|z|, 1).
+__device__ __forceinline__ void box_muller_pair_f(uint32_t site, uint32_t step, uint32_t tag, uint32_t call,
+                                                  uint32_t seed, uint32_t traj, float& z0, float& z1) {
+    uint32_t c[4] = {site, step, tag, call};
+    philox4x32_10(c, seed, traj);
+    const uint64_t m0 = (((uint64_t)c[0] << 32) | c[1]) >> 11;     // u0 = (m0 + 1/2) 2^-53
+    const uint64_t m1 = (((uint64_t)c[2] << 32) | c[3]) >> 11;     // u1 = (m1 + 1/2) 2^-53
+    float lu;
+    if (m0 < (1ull << 52)) lu = logf(((float)m0 + 0.5f) * 0x1.0p-53f);
+    else lu = log1pf(-((float)((1ull << 53) - 1 - m0) + 0.5f) * 0x1.0p-53f);
+    const float rr = sqrtf(-2.0f * lu);
+    const int64_t d = 2 * (int64_t)m1 + 1 - (1ll << 53);            // x = d 2^-53
+    float s, co;
+    sincospif((float)d * 0x1.0p-53f, &s, &co);
+    z0 = -rr * co;
+    z1 = -rr * s;
+}
+
+// Projection of a near-unitary 3x3 matrix onto SU(3): Gram-Schmidt on rows 0
+// and 1, row 2 = conj(row0 x row1) (unit determinant).  The periodic
+// reunitarisation of the mixed-precision form.
+__device__ __forceinline__ m3 su3_project(const m3& M) {
+    m3 R;
+    double n0 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) n0 += M.m[k].x * M.m[k].x + M.m[k].y * M.m[k].y;
+    n0 = 1.0 / sqrt(n0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) R.m[k] = cscale(M.m[k], n0);
+    cplx p = mk(0.0, 0.0);     // <row0, row1>
+#pragma unroll
+    for (int k = 0; k < 3; k++) p = cadd(p, cmulc(M.m[3 + k], R.m[k]));
+    double n1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        R.m[3 + k] = csub(M.m[3 + k], cmul(p, R.m[k]));
+        n1 += R.m[3 + k].x * R.m[3 + k].x + R.m[3 + k].y * R.m[3 + k].y;
+    }
+    n1 = 1.0 / sqrt(n1);
+#pragma unroll
+    for (int k = 0; k < 3; k++) R.m[3 + k] = cscale(R.m[3 + k], n1);
+    R.m[6] = conjg(csub(cmul(R.m[1], R.m[5]), cmul(R.m[2], R.m[4])));
+    R.m[7] = conjg(csub(cmul(R.m[2], R.m[3]), cmul(R.m[0], R.m[5])));
+    R.m[8] = conjg(csub(cmul(R.m[0], R.m[4]), cmul(R.m[1], R.m[3])));
+    return R;
+}
+
 // ---------------------------------------------------------------------------
 // Radix-2/4/8 DFT kernels in registers.  DIR = -1: exp(-2 pi i jk/R) (forward),
 // DIR = +1: exp(+2 pi i jk/R) (backward).
 // ---------------------------------------------------------------------------
-template <int DIR>
-__device__ __forceinline__ cplx rot_q(cplx a) { return DIR < 0 ? cmulmi(a) : cmuli(a); }  // a * e^{DIR i pi/2}
+template <int DIR, typename C>
+__device__ __forceinline__ C rot_q(C a) { return DIR < 0 ? cmulmi(a) : cmuli(a); }  // a * e^{DIR i pi/2}
 
-template <int R, int DIR>
-__device__ __forceinline__ void dft_small(cplx* v) {
+template <int R, int DIR, typename C>
+__device__ __forceinline__ void dft_small(C* v) {
+    typedef decltype(v[0].x) T;
     if constexpr (R == 2) {
-        const cplx a = v[0], b = v[1];
+        const C a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     } else if constexpr (R == 4) {
-        const cplx t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-        const cplx t2 = cadd(v[1], v[3]), t3 = rot_q<DIR>(csub(v[1], v[3]));
+        const C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const C t2 = cadd(v[1], v[3]), t3 = rot_q<DIR>(csub(v[1], v[3]));
         v[0] = cadd(t0, t2);
         v[2] = csub(t0, t2);
         v[1] = cadd(t1, t3);
         v[3] = csub(t1, t3);
     } else {   // R == 8: two radix-4 halves and the e^{DIR i pi k/4} twiddles
-        cplx e[4] = {v[0], v[2], v[4], v[6]};
-        cplx o[4] = {v[1], v[3], v[5], v[7]};
+        C e[4] = {v[0], v[2], v[4], v[6]};
+        C o[4] = {v[1], v[3], v[5], v[7]};
         dft_small<4, DIR>(e);
         dft_small<4, DIR>(o);
-        const double h = 0.70710678118654752440084436210485;
+        const T h = (T)0.70710678118654752440084436210485;
         // o1 *= (1 + DIR i)/sqrt2 ; o2 *= DIR i ; o3 *= (-1 + DIR i)/sqrt2
-        const cplx o1 = DIR < 0 ? mk((o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h)
-                                : mk((o[1].x - o[1].y) * h, (o[1].y + o[1].x) * h);
-        const cplx o2 = rot_q<DIR>(o[2]);
-        const cplx o3 = DIR < 0 ? mk((-o[3].x + o[3].y) * h, (-o[3].y - o[3].x) * h)
-                                : mk((-o[3].x - o[3].y) * h, (-o[3].y + o[3].x) * h);
+        C o1, o3;
+        if (DIR < 0) {
+            o1.x = (o[1].x + o[1].y) * h; o1.y = (o[1].y - o[1].x) * h;
+            o3.x = (-o[3].x + o[3].y) * h; o3.y = (-o[3].y - o[3].x) * h;
+        } else {
+            o1.x = (o[1].x - o[1].y) * h; o1.y = (o[1].y + o[1].x) * h;
+            o3.x = (-o[3].x - o[3].y) * h; o3.y = (-o[3].y + o[3].x) * h;
+        }
+        const C o2 = rot_q<DIR>(o[2]);
         v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
         v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
         v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
@@ -222,7 +301,7 @@ __device__ __forceinline__ void dft_small(cplx* v) {
 // lpos<ES, SS, PS>(n, s) = (n + (n >> PS)) ES + s SS: one padding slot every
 // 2^PS elements (PS = 0: none) keeps the strided Stockham writes off common
 // LDS banks (PS = 4 for rows, ES = 1; PS = 3 for column tiles, ES > 1).
-// tw[m] = exp(-2 pi i m / N).  NT threads take part.
+// tw[m] = exp(-2 pi i m / N), in the element type.  NT threads take part.
 // ---------------------------------------------------------------------------
 template <int ES, int SS, int PS>
 __device__ __forceinline__ int lpos(int n, int s) {
@@ -236,15 +315,15 @@ struct padded { static constexpr int value = N + (PS ? (N >> PS) : 0); };
 template <int REM>
 struct radix_of { static constexpr int value = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2); };
 
-template <int N, int P, int R, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
-__device__ __forceinline__ void fft_stage(cplx* buf, const cplx* __restrict__ tw, int tid) {
+template <int N, int P, int R, int NSEQ, int ES, int SS, int NT, int DIR, int PS, typename C>
+__device__ __forceinline__ void fft_stage(C* buf, const C* __restrict__ tw, int tid) {
     constexpr int S = N / R;
     constexpr int NB = S * NSEQ;
     constexpr int PER = (NB + NT - 1) / NT;
     // With power-of-two N, P, R and padding, lpos(j + r S) = lpos(j) + off(r S) and
     // lpos(b + r P) = lpos(b) + off(r P), off(m) = (m + (m >> PS)) ES: one computed
     // base per butterfly, compile-time offsets per input/output.
-    cplx v[PER][R];
+    C v[PER][R];
     int jj[PER], ss[PER];
 #pragma unroll
     for (int q = 0; q < PER; q++) {
@@ -255,7 +334,7 @@ __device__ __forceinline__ void fft_stage(cplx* buf, const cplx* __restrict__ tw
         jj[q] = j;
         ss[q] = s;
         if (NB % NT == 0 || b < NB) {
-            const cplx* rb = buf + lpos<ES, SS, PS>(j, s);
+            const C* rb = buf + lpos<ES, SS, PS>(j, s);
 #pragma unroll
             for (int r = 0; r < R; r++) v[q][r] = rb[lpos<ES, 0, PS>(r * S, 0)];
         }
@@ -271,13 +350,13 @@ __device__ __forceinline__ void fft_stage(cplx* buf, const cplx* __restrict__ tw
                 constexpr int STRIDE = N / (P * R);
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    cplx w = tw[r * k * STRIDE];
+                    C w = tw[r * k * STRIDE];
                     if (DIR > 0) w.y = -w.y;
                     v[q][r] = cmul(v[q][r], w);
                 }
             }
             dft_small<R, DIR>(v[q]);
-            cplx* wb = buf + lpos<ES, SS, PS>((j - k) * R + k, s);
+            C* wb = buf + lpos<ES, SS, PS>((j - k) * R + k, s);
 #pragma unroll
             for (int r = 0; r < R; r++) wb[lpos<ES, 0, PS>(r * P, 0)] = v[q][r];
         }
@@ -287,7 +366,8 @@ __device__ __forceinline__ void fft_stage(cplx* buf, const cplx* __restrict__ tw
 
 template <int N, int P, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
 struct fft_stages {
-    static __device__ __forceinline__ void run(cplx* buf, const cplx* __restrict__ tw, int tid) {
+    template <typename C>
+    static __device__ __forceinline__ void run(C* buf, const C* __restrict__ tw, int tid) {
         constexpr int R = radix_of<N / P>::value;
         fft_stage<N, P, R, NSEQ, ES, SS, NT, DIR, PS>(buf, tw, tid);
         fft_stages<N, P * R, NSEQ, ES, SS, NT, DIR, PS>::run(buf, tw, tid);
@@ -295,11 +375,13 @@ struct fft_stages {
 };
 template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
 struct fft_stages<N, N, NSEQ, ES, SS, NT, DIR, PS> {
-    static __device__ __forceinline__ void run(cplx*, const cplx* __restrict__, int) {}
+    template <typename C>
+    static __device__ __forceinline__ void run(C*, const C* __restrict__, int) {}
 };
 
-template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS = 0>
-__device__ __forceinline__ void lds_fft(cplx* buf, const cplx* __restrict__ tw, int tid) {
+// C = double2 (FP64 form) or float2 (mixed form), deduced from buf and tw
+template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS = 0, typename C>
+__device__ __forceinline__ void lds_fft(C* buf, const C* __restrict__ tw, int tid) {
     fft_stages<N, 1, NSEQ, ES, SS, NT, DIR, PS>::run(buf, tw, tid);
 }
 

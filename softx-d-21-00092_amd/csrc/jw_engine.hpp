@@ -1,4 +1,6 @@
-// jw_engine.hip -- fused generator-basis Langevin step for 64 <= N <= 2048.
+// jw_engine.hpp -- fused generator-basis Langevin step for 64 <= N <= 2048,
+// templated on the storage type C (double2: FP64 form, float2: mixed form);
+// instantiated by jw_step_f64.hip and jw_step_f32.hip.
 //
 // One step (main_optimized.cpp:243-304 in the generator basis) as six passes
 // over HBM instead of the reference's 8 nine-component FFTs and 5 site loops:
@@ -25,6 +27,8 @@
 //            four plane pairs in one workgroup); row passes touch BF-row
 //            segments whose rows are processed on one XCD at neighbouring times.
 // U always stays row-major.
+#pragma once
+
 #include "jw_common.hpp"
 
 namespace jw {
@@ -38,18 +42,25 @@ struct RowE {
     static constexpr int SPT = N / NT;                  // sites per thread
     static_assert(SPT >= 1 && SPT * NT == N, "row pass needs N a multiple of its thread count");
 };
-// Column passes hold the X and Y tiles of one W-column strip together in LDS
-// (2 W N complex values plus one padding row in eight, 144 KB at most):
-// W = min(16, 4096 / N).
-template <int N>
+// Row-major column passes hold the X and Y tiles of one W-column strip together
+// in LDS (2 W N complex values plus one padding row in eight, 144 KB at most):
+// W = min(16, 4096 / N) for FP64, twice that for FP32.
+template <int N, typename C = double2>
 struct ColE {
-    static constexpr int W = (4096 / N > 16) ? 16 : 4096 / N;
+    static constexpr int W0 = 4096 * (int)(sizeof(double2) / sizeof(C)) / N;    // 128 KB of tiles
+    static constexpr int W = W0 > 16 ? 16 : W0;
     static constexpr int ELEMS = 2 * W * N;
     static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 1024) ? 1024 : ELEMS / 8);
     static constexpr int PER = ELEMS / NT;          // elements per thread
     static constexpr int PAIRS = (W * N) / NT;      // (X, Y) element pairs per thread
     static_assert(W >= 1 && W <= N && N % W == 0 && PER * NT == ELEMS && PAIRS * NT == W * N, "column tiling");
 };
+
+// minimum waves per SIMD requested of the row passes: the FP32 rows (4 padded
+// rows, 70 KB of LDS at N = 2048) fit two 512-thread workgroups per CU when
+// they stay within 128 VGPRs
+template <typename C>
+struct RowWaves { static constexpr int value = sizeof(C) == sizeof(double2) ? 1 : 4; };
 
 // strip index for block b so that the strips on one XCD (blocks b, b+8, ...) are adjacent
 __device__ __forceinline__ int xcd_strip(int b, int nstrips) {
@@ -95,27 +106,28 @@ __device__ __forceinline__ int opaque_v(int v) {
 // ---------------------------------------------------------------------------
 // row pass 1: noise -> xi row spectra (planes 0..3 xi_x, 4..7 xi_y)
 // ---------------------------------------------------------------------------
-template <int N, int BF>
-__global__ __launch_bounds__(RowE<N>::NT) void k_row_noise_fwd(cplx* __restrict__ Xi, size_t V, NoiseSrc ns,
+template <int N, int BF, typename C>
+__global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_noise_fwd(C* __restrict__ Xi, size_t V, NoiseSrc ns,
                                                               const double* __restrict__ sa_pos,
-                                                              const cplx* __restrict__ tw) {
+                                                              const C* __restrict__ tw) {
+    typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
-    __shared__ cplx lds[4 * PN];
+    __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
     const int x = row_of_block<N, BF>(blockIdx.x);
     const size_t row0 = (size_t)x * N;
-    double m[SPT][8];
+    T m[SPT][8];
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
         const int y = tid + q * NT;
-        double z[16];
-        normals16(ns, row0 + y, z);
+        T z[16];
+        normals16_t(ns, row0 + y, z);
         const double s = sa_pos ? sa_pos[row0 + y] : 1.0;
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mk(z[4 * p] * s, z[4 * p + 2] * s);
+        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mkc<C>(z[4 * p] * s, z[4 * p + 2] * s);
 #pragma unroll
-        for (int k = 0; k < 8; k++) m[q][k] = z[2 * k + 1] * s;
+        for (int k = 0; k < 8; k++) m[q][k] = (T)(z[2 * k + 1] * s);
     }
     __syncthreads();
     lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
@@ -127,7 +139,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_noise_fwd(cplx* __restrict_
 #pragma unroll
     for (int q = 0; q < SPT; q++)
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = mk(m[q][2 * p], m[q][2 * p + 1]);
+        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = mkc<C>(m[q][2 * p], m[q][2 * p + 1]);
     __syncthreads();
     lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
 #pragma unroll
@@ -140,35 +152,38 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_noise_fwd(cplx* __restrict_
 // row pass 2: V_i = U xi_i U^dagger -> V row spectra (planes 0..3 V_x, 4..7 V_y),
 // xi regenerated from the noise source (white noise)
 // ---------------------------------------------------------------------------
-template <int N, int BF>
-__global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(cplx* __restrict__ Vv, const cplx* __restrict__ U,
+template <int N, int BF, typename C>
+__global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(C* __restrict__ Vv, const C* __restrict__ U,
                                                              size_t V, NoiseSrc ns, const double* __restrict__ sa_pos,
-                                                             const cplx* __restrict__ tw) {
+                                                             const C* __restrict__ tw) {
+    typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
-    __shared__ cplx lds[4 * PN];
+    __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
     const int x = row_of_block<N, BF>(blockIdx.x);
     const size_t row0 = (size_t)x * N;
-    double vy[SPT][8];
+    T vy[SPT][8];      // in the element type: they go to LDS in it
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
         const int y = tid + q * NT;
         const size_t i = row0 + y;
         double hx[8], hy[8];
-        double z[16];
-        normals16(ns, i, z);
+        T z[16];
+        normals16_t(ns, i, z);
         const double s = sa_pos ? sa_pos[i] : 1.0;
 #pragma unroll
         for (int k = 0; k < 8; k++) { hx[k] = z[2 * k] * s; hy[k] = z[2 * k + 1] * s; }
         m3 u;
 #pragma unroll
-        for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
-        double vx[8];
+        for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
+        double vx[8], v2[8];
         uxiu_gen(u, hx, vx);
-        uxiu_gen(u, hy, vy[q]);
+        uxiu_gen(u, hy, v2);
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mk(vx[2 * p], vx[2 * p + 1]);
+        for (int k = 0; k < 8; k++) vy[q][k] = (T)v2[k];
+#pragma unroll
+        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mkc<C>(vx[2 * p], vx[2 * p + 1]);
     }
     __syncthreads();
     lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
@@ -180,7 +195,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(cplx* __restrict__
 #pragma unroll
     for (int q = 0; q < SPT; q++)
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = mk(vy[q][2 * p], vy[q][2 * p + 1]);
+        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = mkc<C>(vy[q][2 * p], vy[q][2 * p + 1]);
     __syncthreads();
     lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
 #pragma unroll
@@ -197,13 +212,13 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(cplx* __restrict__
 // overwrite its own xi~ slots in LDS before the forward row FFT; U is read once
 // per batch (the second read is served by the caches).
 // ---------------------------------------------------------------------------
-template <int N, int BF>
-__global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_corr(cplx* __restrict__ Vv, const cplx* __restrict__ U,
-                                                              const cplx* __restrict__ Xs, size_t V,
-                                                              const cplx* __restrict__ tw) {
+template <int N, int BF, typename C>
+__global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_corr(C* __restrict__ Vv, const C* __restrict__ U,
+                                                              const C* __restrict__ Xs, size_t V,
+                                                              const C* __restrict__ tw) {
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;
-    __shared__ cplx lds[4 * PN];
+    __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
     const int x = row_of_block<N, BF>(blockIdx.x);
     const size_t row0 = (size_t)x * N;
@@ -224,17 +239,17 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_corr(cplx* __restrict_
             double h[8];
 #pragma unroll
             for (int p = 0; p < 4; p++) {
-                const cplx a = lds[lpos<1, PN, 4>(y, p)];
+                const C a = lds[lpos<1, PN, 4>(y, p)];
                 h[2 * p] = a.x;
                 h[2 * p + 1] = a.y;
             }
             m3 u;
 #pragma unroll
-            for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+            for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
             double v[8];
             uxiu_gen(u, h, v);
 #pragma unroll
-            for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mk(v[2 * p], v[2 * p + 1]);
+            for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mkc<C>(v[2 * p], v[2 * p + 1]);
         }
         __syncthreads();
         lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
@@ -256,16 +271,17 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_corr(cplx* __restrict_
 // LDS row x holds [X(x, c0..c0+W-1), Y(x, c0..c0+W-1)]: 2W sequences of length N
 // with element stride 2W.
 // ---------------------------------------------------------------------------
-template <int N, int CORR>
-__global__ __launch_bounds__(ColE<N>::NT) void k_col_kmult(cplx* __restrict__ Z, cplx* __restrict__ X,
-                                                          cplx* __restrict__ Y, size_t V, int sqrt_coupling,
+template <int N, int CORR, typename C>
+__global__ __launch_bounds__((ColE<N, C>::NT)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
+                                                          C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                           const double* __restrict__ q4,
                                                           const double* __restrict__ sb,
                                                           const double* __restrict__ sa,
-                                                          const cplx* __restrict__ tw) {
-    constexpr int W = ColE<N>::W, NT = ColE<N>::NT, PER = ColE<N>::PER, PAIRS = ColE<N>::PAIRS;
+                                                          const C* __restrict__ tw) {
+    typedef typename scalar_of<C>::type T;
+    constexpr int W = ColE<N, C>::W, NT = ColE<N, C>::NT, PER = ColE<N, C>::PER, PAIRS = ColE<N, C>::PAIRS;
     constexpr int S2 = 2 * W;
-    __shared__ cplx lds[padded<N, 3>::value * S2];
+    __shared__ C lds[padded<N, 3>::value * S2];
     const int tid = threadIdx.x;
     const int strip = xcd_strip(blockIdx.x, N / W);
     const int col0 = strip * W;
@@ -282,13 +298,13 @@ __global__ __launch_bounds__(ColE<N>::NT) void k_col_kmult(cplx* __restrict__ Z,
     __syncthreads();
     lds_fft<N, S2, S2, 1, NT, -1, 3>(lds, tw, tid);
     // pair (x, c): X^ in slot c of LDS row x, Y^ in slot W + c
-    cplx zreg[CORR ? PAIRS : 1];
+    C zreg[CORR ? PAIRS : 1];
 #pragma unroll
     for (int q = 0; q < PAIRS; q++) {
         const int pr = tid + q * NT;
         const int kx = pr / W, c = pr % W, ky = col0 + c;
         const size_t k = (size_t)kx * N + ky;
-        cplx vx = lds[lpos<S2, 1, 3>(kx, c)], vy = lds[lpos<S2, 1, 3>(kx, W + c)];
+        C vx = lds[lpos<S2, 1, 3>(kx, c)], vy = lds[lpos<S2, 1, 3>(kx, W + c)];
         double fx = 0.0, fy = 0.0;
         if (kx != 0 || ky != 0) {
             const double ph = q4[kx] + q4[ky];
@@ -297,13 +313,13 @@ __global__ __launch_bounds__(ColE<N>::NT) void k_col_kmult(cplx* __restrict__ Z,
             fy = -2.0 * M_PI * cc * sb[ky] / ph * invV;
         }
         if (CORR) {
-            const double s = sa[k];
+            const T s = (T)sa[k];
             vx = cscale(vx, s);
             vy = cscale(vy, s);
-            lds[lpos<S2, 1, 3>(kx, c)] = cscale(vx, invV);
-            lds[lpos<S2, 1, 3>(kx, W + c)] = cscale(vy, invV);
+            lds[lpos<S2, 1, 3>(kx, c)] = cscale(vx, (T)invV);
+            lds[lpos<S2, 1, 3>(kx, W + c)] = cscale(vy, (T)invV);
         }
-        const cplx z = mk(-(fx * vx.y + fy * vy.y), fx * vx.x + fy * vy.x);   // i (fx vx + fy vy)
+        const C z = mkc<C>(-(fx * vx.y + fy * vy.y), fx * vx.x + fy * vy.x);   // i (fx vx + fy vy)
         if (CORR) zreg[q] = z;
         else lds[lpos<S2, 1, 3>(kx, c)] = z;
     }
@@ -347,35 +363,40 @@ __global__ __launch_bounds__(ColE<N>::NT) void k_col_kmult(cplx* __restrict__ Z,
 // x-block e / 8, slot e mod 8 = c BF + x mod BF: consecutive threads read
 // consecutive 16-byte words.  sa16 holds sa(k) in 16-row blocks.
 // ---------------------------------------------------------------------------
-template <int N, int BF>
+template <int N, int BF, typename C>
 struct ColB {
-    static constexpr int CPB = 8 / BF;                 // columns per workgroup
+    static constexpr int SEG = 128 / (int)sizeof(C);   // elements per 128-byte segment
+    static constexpr int CPB = SEG / BF;               // columns per workgroup
     static constexpr int ELEMS = CPB * N;
     static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 512) ? 512 : ELEMS / 8);
     static constexpr int PER = ELEMS / NT;             // elements per thread
-    static_assert((BF == 4 || BF == 8) && PER * NT == ELEMS && N % 16 == 0, "blocked column tiling");
+    static_assert((BF == 4 || BF == 8) && CPB >= 1 && PER * NT == ELEMS && N % 16 == 0 && N % CPB == 0,
+                  "blocked column tiling");
 };
 
-template <int N, int BF, int CORR>
-__global__ __launch_bounds__((ColB<N, BF>::NT)) void k_col_kmult_blk(cplx* __restrict__ Z, cplx* __restrict__ X,
-                                                                  cplx* __restrict__ Y, size_t V, int sqrt_coupling,
-                                                                  const double* __restrict__ q4,
-                                                                  const double* __restrict__ sb,
-                                                                  const double* __restrict__ sa16,
-                                                                  const cplx* __restrict__ tw) {
-    constexpr int CPB = ColB<N, BF>::CPB, NT = ColB<N, BF>::NT, PER = ColB<N, BF>::PER;
+template <int N, int BF, int CORR, typename C>
+__global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __restrict__ Z, C* __restrict__ X,
+                                                                     C* __restrict__ Y, size_t V, int sqrt_coupling,
+                                                                     const double* __restrict__ q4,
+                                                                     const double* __restrict__ sb,
+                                                                     const double* __restrict__ sa16,
+                                                                     const C* __restrict__ tw) {
+    typedef typename scalar_of<C>::type T;
+    constexpr int SEG = ColB<N, BF, C>::SEG, CPB = ColB<N, BF, C>::CPB;
+    constexpr int NT = ColB<N, BF, C>::NT, PER = ColB<N, BF, C>::PER;
     constexpr int PN = padded<N, 4>::value;
-    __shared__ cplx lds[CPB * PN];
+    __shared__ C lds[CPB * PN];
     const int tid = threadIdx.x;
     const int y0 = blockIdx.x * CPB;
     const double invV = 1.0 / ((double)N * (double)N);
-    // global offset within a plane / LDS slot of loaded element e
-    auto goff = [&](int e) -> size_t { return (size_t)(e >> 3) * ((size_t)N * BF) + (size_t)y0 * BF + (e & 7); };
+    // global offset within a plane / LDS slot of loaded element e: x-block
+    // e / SEG, slot e mod SEG = c BF + x mod BF
+    auto goff = [&](int e) -> size_t { return (size_t)(e / SEG) * ((size_t)N * BF) + (size_t)y0 * BF + (e % SEG); };
     auto lofs = [&](int e) -> int {
-        const int sl = e & 7;
-        return lpos<1, PN, 4>((e >> 3) * BF + sl % BF, sl / BF);
+        const int sl = e % SEG;
+        return lpos<1, PN, 4>((e / SEG) * BF + sl % BF, sl / BF);
     };
-    auto load = [&](const cplx* __restrict__ P) {
+    auto load = [&](const C* __restrict__ P) {
         const int t = opaque_v(tid);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
@@ -383,7 +404,7 @@ __global__ __launch_bounds__((ColB<N, BF>::NT)) void k_col_kmult_blk(cplx* __res
             lds[lofs(e)] = P[goff(e)];
         }
     };
-    auto store = [&](cplx* __restrict__ P) {
+    auto store = [&](C* __restrict__ P) {
         const int t = opaque_v(tid);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
@@ -395,8 +416,8 @@ __global__ __launch_bounds__((ColB<N, BF>::NT)) void k_col_kmult_blk(cplx* __res
     // i f_d P^ into acc for this thread's pairs (kx = pr mod N, column c = pr / N)
     // with f_d = -2 pi cc sb[k_d] / phat^2 / V, zero at k = 0; CORR: scale P^
     // by sa and write F_col^-1[sa P^ / V] back over P
-    auto direction = [&](cplx* __restrict__ P, cplx acc[PER], int d) {
-        const cplx* twd = opaque(tw);
+    auto direction = [&](C* __restrict__ P, C acc[PER], int d) {
+        const C* twd = opaque(tw);
         const double* q4d = opaque(q4);
         const double* sbd = opaque(sb);
         const double* sad = opaque(sa16);
@@ -410,10 +431,10 @@ __global__ __launch_bounds__((ColB<N, BF>::NT)) void k_col_kmult_blk(cplx* __res
             const int kx = pr % N, c = pr / N, ky = y0 + c;
             const int l = lpos<1, PN, 4>(kx, c);
             const size_t k16 = ((size_t)(kx >> 4) * N + ky) * 16 + (kx & 15);
-            cplx v = lds[l];
+            C v = lds[l];
             if (CORR) {
-                v = cscale(v, sad[k16]);
-                lds[l] = cscale(v, invV);
+                v = cscale(v, (T)sad[k16]);
+                lds[l] = cscale(v, (T)invV);
             }
             double f = 0.0;
             if (kx != 0 || ky != 0) {
@@ -421,8 +442,8 @@ __global__ __launch_bounds__((ColB<N, BF>::NT)) void k_col_kmult_blk(cplx* __res
                 const double cc = sqrt_coupling ? sad[k16] : 1.0;
                 f = -2.0 * M_PI * cc * sbd[d ? ky : kx] / ph * invV;
             }
-            if (d == 0) acc[q] = mk(-(f * v.y), f * v.x);
-            else acc[q] = mk(acc[q].x - f * v.y, acc[q].y + f * v.x);
+            if (d == 0) acc[q] = mkc<C>(-(f * v.y), f * v.x);
+            else acc[q] = mkc<C>(acc[q].x - f * v.y, acc[q].y + f * v.x);
         }
         __syncthreads();
         if (CORR) {
@@ -434,7 +455,7 @@ __global__ __launch_bounds__((ColB<N, BF>::NT)) void k_col_kmult_blk(cplx* __res
     // plane pair p = blockIdx.y (one pass over the pairs would let the compiler
     // hoist every twiddle and kernel factor out of the loop into registers)
     const size_t po = (size_t)blockIdx.y * V;
-    cplx acc[PER];
+    C acc[PER];
     direction(X + po, acc, 0);
     direction(Y + po, acc, 1);
     const int t = opaque_v(tid);
@@ -448,32 +469,22 @@ __global__ __launch_bounds__((ColB<N, BF>::NT)) void k_col_kmult_blk(cplx* __res
     store(Z + po);
 }
 
-__global__ __launch_bounds__(256) void k_block16(double* __restrict__ out, const double* __restrict__ in, int N) {
-    const size_t V = (size_t)N * N;
-    GRID_STRIDE(i, V) {
-        const int kx = (int)(i / N), ky = (int)(i % N);
-        out[((size_t)(kx >> 4) * N + ky) * 16 + (kx & 15)] = in[i];
-    }
-}
-
-hipError_t block16(jw_ctx* c, double* out, const double* in) {
-    if (c->N % 16) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_block16, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, out, in, c->N);
-    return hipGetLastError();
-}
 
 // ---------------------------------------------------------------------------
 // row pass 3: inverse row FFTs of the A and B column-inverse spectra and the
 // update U <- exp(i s B) U exp(-i s A) (field.h:1966-1996) in one pass.
-// A's coefficients are kept in registers, B's stay in LDS.
+// A's coefficients are kept in registers, B's stay in LDS.  The site algebra
+// (exponentials, products, the optional projection onto SU(3)) is FP64 in both
+// forms; U is stored in the element type C.
 // ---------------------------------------------------------------------------
-template <int N, int BF>
-__global__ __launch_bounds__((RowE<N, 4>::NT)) void k_row_update(cplx* __restrict__ U, const cplx* __restrict__ ZA,
-                                                           const cplx* __restrict__ ZB, size_t V, double ss,
-                                                           const cplx* __restrict__ tw) {
+template <int N, int BF, typename C>
+__global__ __launch_bounds__((RowE<N, 4>::NT)) void k_row_update(C* __restrict__ U, const C* __restrict__ ZA,
+                                                              const C* __restrict__ ZB, size_t V, double ss,
+                                                              int reunit, const C* __restrict__ tw) {
+    typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N, 4>::NT, SPT = RowE<N, 4>::SPT;
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
-    __shared__ cplx lds[4 * PN];
+    __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
     const int x = row_of_block<N, BF>(blockIdx.x);
     const size_t row0 = (size_t)x * N;
@@ -483,12 +494,12 @@ __global__ __launch_bounds__((RowE<N, 4>::NT)) void k_row_update(cplx* __restric
         for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZA[p * V + lay<N, BF>(x, tid + q * NT)];
     __syncthreads();
     lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
-    double ha[SPT][8];
+    T ha[SPT][8];
 #pragma unroll
     for (int q = 0; q < SPT; q++)
 #pragma unroll
         for (int p = 0; p < 4; p++) {
-            const cplx a = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+            const C a = lds[lpos<1, PN, 4>(tid + q * NT, p)];
             ha[q][2 * p] = a.x;
             ha[q][2 * p + 1] = a.y;
         }
@@ -506,104 +517,111 @@ __global__ __launch_bounds__((RowE<N, 4>::NT)) void k_row_update(cplx* __restric
         double hb[8];
 #pragma unroll
         for (int p = 0; p < 4; p++) {
-            const cplx b = lds[lpos<1, PN, 4>(y, p)];
+            const C b = lds[lpos<1, PN, 4>(y, p)];
             hb[2 * p] = b.x;
             hb[2 * p + 1] = b.y;
         }
         m3 u;
 #pragma unroll
-        for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
-        const m3 w = mmul(exp_i_gen(hb, ss), u);
-        const m3 r = mmul(w, exp_i_gen(ha[q], -ss));
+        for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
+        double hd[8];
 #pragma unroll
-        for (int k = 0; k < 9; k++) U[k * V + i] = r.m[k];
+        for (int k = 0; k < 8; k++) hd[k] = ha[q][k];
+        const m3 w = mmul(exp_i_gen(hb, ss), u);
+        m3 r = mmul(w, exp_i_gen(hd, -ss));
+        if (reunit) r = su3_project(r);
+#pragma unroll
+        for (int k = 0; k < 9; k++) U[k * V + i] = mkc<C>(r.m[k].x, r.m[k].y);
     }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int N, int BF>
-static hipError_t col_pass(jw_ctx* c, cplx* Z, cplx* X, cplx* Y, const double* sa_k, int sqrt_coupling, int corr) {
+template <typename C>
+static const C* twiddles(const jw_ctx* c) {
+    if constexpr (sizeof(C) == sizeof(cplx)) return c->tw;
+    else return c->twf;
+}
+
+template <int N, int BF, typename C>
+static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int sqrt_coupling, int corr) {
     const size_t V = c->V;
+    const C* tw = twiddles<C>(c);
     if constexpr (BF == 0) {
-        constexpr int W = ColE<N>::W;
+        constexpr int W = ColE<N, C>::W, NT = ColE<N, C>::NT;
         if (corr)
-            hipLaunchKernelGGL((k_col_kmult<N, 1>), dim3(N / W, 4), dim3(ColE<N>::NT), 0, c->stream, Z, X, Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
+            hipLaunchKernelGGL((k_col_kmult<N, 1, C>), dim3(N / W, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
+                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
         else
-            hipLaunchKernelGGL((k_col_kmult<N, 0>), dim3(N / W, 4), dim3(ColE<N>::NT), 0, c->stream, Z, X, Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
+            hipLaunchKernelGGL((k_col_kmult<N, 0, C>), dim3(N / W, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
+                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
     } else {
-        constexpr int CPB = ColB<N, BF>::CPB, NT = ColB<N, BF>::NT;
+        constexpr int CPB = ColB<N, BF, C>::CPB, NT = ColB<N, BF, C>::NT;
         if (corr)
-            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1>), dim3(N / CPB, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
+            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1, C>), dim3(N / CPB, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
+                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
         else
-            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 0>), dim3(N / CPB, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, c->tw);
+            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 0, C>), dim3(N / CPB, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
+                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
     }
     return hipGetLastError();
 }
 
-template <int N, int BF>
-static hipError_t step_fused_n(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
-                               const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr,
-                               double sqrt_step, const int* kcls) {
+template <int N, int BF, typename C>
+static hipError_t step_fused_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, const StepArgs& a, const int* kcls) {
     const size_t V = c->V;
+    const C* tw = twiddles<C>(c);
     hipError_t e;
     {
         KTimer t(c, kcls[0]);
-        hipLaunchKernelGGL((k_row_noise_fwd<N, BF>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Xi, V, ns, sa_pos,
-                           c->tw);
+        hipLaunchKernelGGL((k_row_noise_fwd<N, BF, C>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Xi, V, a.ns,
+                           a.sa_pos, tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[1]);
-        if ((e = col_pass<N, BF>(c, ZA, Xi, Xi + 4 * V, sa_k, sqrt_coupling, corr)) != hipSuccess) return e;
+        if ((e = col_pass<N, BF>(c, ZA, Xi, Xi + 4 * V, a.sa_k, a.sqrt_coupling, a.corr)) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[2]);
-        if (corr)
-            hipLaunchKernelGGL((k_row_uxiu_corr<N, BF>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, Xi, V,
-                               c->tw);
+        if (a.corr)
+            hipLaunchKernelGGL((k_row_uxiu_corr<N, BF, C>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, Xi, V,
+                               tw);
         else
-            hipLaunchKernelGGL((k_row_uxiu_fwd<N, BF>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, V, ns,
-                               sa_pos, c->tw);
+            hipLaunchKernelGGL((k_row_uxiu_fwd<N, BF, C>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, V, a.ns,
+                               a.sa_pos, tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[1]);
-        if ((e = col_pass<N, BF>(c, ZB, Vv, Vv + 4 * V, sa_k, sqrt_coupling, 0)) != hipSuccess) return e;
+        if ((e = col_pass<N, BF>(c, ZB, Vv, Vv + 4 * V, a.sa_k, a.sqrt_coupling, 0)) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[3]);
-        hipLaunchKernelGGL((k_row_update<N, BF>), dim3(N), dim3(RowE<N, 4>::NT), 0, c->stream, U, ZA, ZB, V, sqrt_step,
-                           c->tw);
+        hipLaunchKernelGGL((k_row_update<N, BF, C>), dim3(N), dim3(RowE<N, 4>::NT), 0, c->stream, U, ZA, ZB, V,
+                           a.sqrt_step, a.reunit, tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-bool fused_supported(int N) { return N >= 64 && N <= 2048; }
-
-template <int N>
-static hipError_t step_fused_layout(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
-                                    const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr,
-                                    double sqrt_step, const int* kcls) {
+template <int N, typename C>
+static hipError_t step_fused_layout(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, const StepArgs& a,
+                                    const int* kcls) {
     switch (c->layout) {
-        case 0: return step_fused_n<N, 0>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
-        case 4: return step_fused_n<N, 4>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
-        case 8: return step_fused_n<N, 8>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
+        case 0: return step_fused_n<N, 0>(c, U, Xi, Vv, ZA, ZB, a, kcls);
+        case 4: return step_fused_n<N, 4>(c, U, Xi, Vv, ZA, ZB, a, kcls);
+        case 8: return step_fused_n<N, 8>(c, U, Xi, Vv, ZA, ZB, a, kcls);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
-                      const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr, double sqrt_step) {
+template <typename C>
+static hipError_t step_fused_t(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, const StepArgs& a) {
     static const int kcls[4] = {K_ROW_NOISE, K_COL_KMULT, K_ROW_UXIU, K_ROW_UPDATE};
 #define JW_FUSED_CASE(n) \
-    case n: return step_fused_layout<n>(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, sqrt_coupling, corr, sqrt_step, kcls);
+    case n: return step_fused_layout<n>(c, U, Xi, Vv, ZA, ZB, a, kcls);
     switch (c->N) {
         JW_FUSED_CASE(64)
         JW_FUSED_CASE(128)

@@ -19,6 +19,7 @@
 namespace jw {
 
 typedef double2 cplx;
+typedef float2 cplxf;
 
 // Noise source for the generators: injected standard normals (AoS, `per_site`
 // per site, reference draw order) or the Philox stream.
@@ -45,6 +46,7 @@ struct jw_ctx {
     int device;
     hipStream_t stream;
     jw::cplx* tw;          // tw[m] = exp(-2 pi i m / N), m < N
+    float2* twf;           // the same in FP32 (mixed-precision form)
     double* q4;            // q4[k] = 4 pow(sin(pi k / N), 2)   (momenta.h:111)
     double* sb;            // sb[k] = sin(2 pi k / N)           (momenta.h:119-120)
     // sqrt(alpha) table over the lattice, rebuilt when the variant/params change
@@ -110,11 +112,20 @@ hipError_t mv_noise_packed(jw_ctx* c, cplx* R, const NoiseSrc& ns, double sigma)
 hipError_t poisson_packed(jw_ctx* c, cplx* R, double mass, double g);
 hipError_t mul_exp_packed(jw_ctx* c, cplx* U, const cplx* R);
 
-// fused generator-basis step (64 <= N <= 2048)
+// fused generator-basis step (64 <= N <= 2048), FP64 or FP32 storage
 bool fused_supported(int N);
-// sa_k: row-major for layout 0, the 16-row-blocked sa16 otherwise
-hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
-                      const double* sa_pos, const double* sa_k, int sqrt_coupling, int corr, double sqrt_step);
+struct StepArgs {
+    NoiseSrc ns;
+    const double* sa_pos;    // sqrt(alpha) at positions (NOISE coupling) or nullptr
+    const double* sa_k;      // sqrt(alpha) in k: row-major for layout 0, sa16 otherwise; or nullptr
+    int sqrt_coupling;       // kernel factor carries sqrt(alpha) (SQRT coupling)
+    int corr;                // per-mode correlated noise
+    double sqrt_step;        // sqrt(Delta s)
+    int reunit;              // project the updated U onto SU(3)
+};
+hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a);
+hipError_t step_fused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a);
+hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32);
 hipError_t block16(jw_ctx* c, double* out, const double* in);
 
 // observables

@@ -1,0 +1,45 @@
+// jw_step_f64.hip -- the fused step with FP64 storage (the plain form), and the
+// helpers shared by both forms: the 16-row-blocked sqrt(alpha) table and the
+// U conversions of the mixed form.
+#include "jw_engine.hpp"
+
+namespace jw {
+
+bool fused_supported(int N) { return N >= 64 && N <= 2048; }
+
+hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a) {
+    return step_fused_t(c, U, Xi, Vv, ZA, ZB, a);
+}
+
+__global__ __launch_bounds__(256) void k_block16(double* __restrict__ out, const double* __restrict__ in, int N) {
+    const size_t V = (size_t)N * N;
+    GRID_STRIDE(i, V) {
+        const int kx = (int)(i / N), ky = (int)(i % N);
+        out[((size_t)(kx >> 4) * N + ky) * 16 + (kx & 15)] = in[i];
+    }
+}
+
+hipError_t block16(jw_ctx* c, double* out, const double* in) {
+    if (c->N % 16) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_block16, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, out, in, c->N);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// U between the FP64 field and the FP32 state of the mixed-precision form
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_u_to_f32(cplxf* __restrict__ out, const cplx* __restrict__ in, size_t n) {
+    GRID_STRIDE(i, n) out[i] = mkf((float)in[i].x, (float)in[i].y);
+}
+__global__ __launch_bounds__(256) void k_u_to_f64(cplx* __restrict__ out, const cplxf* __restrict__ in, size_t n) {
+    GRID_STRIDE(i, n) out[i] = to_d(in[i]);
+}
+
+hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32) {
+    const size_t n = 9 * c->V;
+    if (to_f32) hipLaunchKernelGGL(k_u_to_f32, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, u32, u64, n);
+    else hipLaunchKernelGGL(k_u_to_f64, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, u64, u32, n);
+    return hipGetLastError();
+}
+
+}  // namespace jw

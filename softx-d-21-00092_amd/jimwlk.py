@@ -38,7 +38,7 @@ def alpha(lambda_L=6.0, mu0_over_lambda=2.5, freeze_c=0.2, beta0=9.0):
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE, "-B", "lib"], check=True)
+    subprocess.run(["make", "-s", "-j8", "-C", HERE, "-B", "lib"], check=True)
 
 
 _lib = None
@@ -66,6 +66,8 @@ def lib():
         L.jw_noise_philox.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.jw_evolve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 ctypes.POINTER(Alpha), ctypes.c_int]
+        L.jw_evolve_mixed.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.POINTER(Alpha), ctypes.c_int, ctypes.c_int]
         L.jw_mv_init.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, _dp]
         L.jw_update_uf.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double]
@@ -205,6 +207,12 @@ class Context:
     # engine
     def evolve(self, U, coupling=FIXED_COUPLING, kernel=SIN_KERNEL, step=1e-4, al=None, nsteps=1):
         _check(lib().jw_evolve(self.h, U.h, coupling, kernel, step, ctypes.byref(al) if al else None, nsteps))
+
+    def evolve_mixed(self, U, coupling=FIXED_COUPLING, kernel=SIN_KERNEL, step=1e-4, al=None, nsteps=1,
+                     reunit_every=10):
+        """jw_evolve_mixed: FP32 fields and FFTs, FP64 exponentials and SU(3) projection."""
+        _check(lib().jw_evolve_mixed(self.h, U.h, coupling, kernel, step, ctypes.byref(al) if al else None, nsteps,
+                                     reunit_every))
 
     def mv_init(self, U, Ny, g, mu, mass, z=None):
         if z is not None:

@@ -1,0 +1,133 @@
+"""GPU tests of the mixed-precision form (jw_evolve_mixed): FP32 fields and
+FFTs, FP64 exponentials and SU(3) projection, against the FP64 form on the
+same noise.
+
+Tolerances:
+  * one step from the same U and injected noise: per-site Frobenius
+    <= 1e-6 (FP32 storage of U alone is ~1e-7);
+  * 1000 steps from the same MV start and Philox stream: dipole S(r) within
+    2e-5 absolute of the FP64 form (the acceptance bound of the mixed form);
+  * the four intermediate layouts agree with each other to FP32 rounding;
+  * with projection, U is unitary with unit determinant to FP32 rounding.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(11)
+
+
+def frob_max(a, b):
+    d = (a - b).reshape(-1, 9)
+    return np.sqrt((abs(d) ** 2).sum(axis=1)).max()
+
+
+def mv_start(jw, c, Ny=10):
+    U = c.field()
+    c.philox(3, 0, 0)
+    c.mv_init(U, Ny, 1.0, 0.25, 1e-4 / 0.25)
+    return U.download()
+
+
+def alpha_for(jw, N, coupling):
+    if coupling not in (0, 1, 4):
+        return None
+    return jw.alpha(lambda_L=0.02 * N, mu0_over_lambda=math.exp(2 * math.pi / (9 * 0.7)))
+
+
+@pytest.mark.parametrize("coupling", [3, 1, 0, 4])
+def test_mixed_one_step_injected(jw, coupling):
+    N = 64
+    c = jw.Context(N)
+    U0 = mv_start(jw, c)
+    z = rng.normal(size=N * N * 16)
+    al = alpha_for(jw, N, coupling)
+    ds = 5e-5 if coupling == 4 else 1e-4
+    U64, U32 = c.field(9, U0), c.field(9, U0)
+    c.inject(z)
+    c.evolve(U64, coupling, 1, ds, al, nsteps=1)
+    c.inject(z)
+    c.evolve_mixed(U32, coupling, 1, ds, al, nsteps=1, reunit_every=0)
+    a, b = U64.download(), U32.download()
+    assert frob_max(a, b) <= 1e-6, frob_max(a, b)
+    assert frob_max(a, U0) > 1e-3
+    c.close()
+
+
+@pytest.mark.parametrize("N,coupling", [(64, 3), (64, 4), (256, 3), (256, 4)])
+def test_mixed_dipole_after_1000_steps(jw, N, coupling):
+    """Same MV start, same Philox stream: S(r) of the two forms after 1000 steps."""
+    c = jw.Context(N)
+    U0 = mv_start(jw, c)
+    al = alpha_for(jw, N, coupling)
+    ds = 5e-5 if coupling == 4 else 1e-4
+    U64, U32 = c.field(9, U0), c.field(9, U0)
+    c.philox(21, 4, 0)
+    c.evolve(U64, coupling, 1, ds, al, nsteps=1000)
+    c.philox(21, 4, 0)
+    c.evolve_mixed(U32, coupling, 1, ds, al, nsteps=1000, reunit_every=10)
+    S64 = c.dipole_from_corr(c.measure_corr(U64))
+    S32 = c.dipole_from_corr(c.measure_corr(U32))
+    S0 = c.dipole_from_corr(c.measure_corr(c.field(9, U0)))
+    err = np.abs(S64 - S32).max()
+    assert err <= 2e-5, err
+    assert np.abs(S64 - S0).max() > 1e-3          # the evolution moved S(r)
+    c.close()
+
+
+@pytest.mark.parametrize("N", [64, 512, 2048])
+@pytest.mark.parametrize("coupling", [3, 4])
+def test_mixed_layouts_agree(jw, N, coupling):
+    c = jw.Context(N)
+    U0 = mv_start(jw, c, Ny=2)
+    al = alpha_for(jw, N, coupling)
+    out = {}
+    for mode in (c.FUSED_ROWMAJOR, c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED):
+        c.set_fused(mode)
+        U = c.field(9, U0)
+        c.philox(8, 0, 0)
+        c.evolve_mixed(U, coupling, 1, 1e-4, al, nsteps=3, reunit_every=0)
+        out[mode] = U.download()
+    ref = out[c.FUSED_ROWMAJOR]
+    for mode in (c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED):
+        assert frob_max(ref, out[mode]) <= 2e-6, mode
+    assert frob_max(ref, U0) > 1e-3
+    c.set_fused(c.FUSED)
+    c.close()
+
+
+def test_mixed_projection_keeps_su3(jw):
+    N = 128
+    c = jw.Context(N)
+    U0 = mv_start(jw, c)
+    for reunit, tol in ((10, 3e-7), (0, None)):
+        U = c.field(9, U0)
+        c.philox(5, 0, 0)
+        c.evolve_mixed(U, 3, 1, 1e-4, None, nsteps=40, reunit_every=reunit)
+        m = U.download().reshape(-1, 3, 3)
+        uni = np.abs(np.einsum("sij,skj->sik", m, m.conj()) - np.eye(3)).max()
+        det = np.abs(np.linalg.det(m) - 1).max()
+        if tol is not None:
+            assert uni <= tol and det <= tol, (uni, det)
+        else:
+            assert uni < 1e-4      # without projection: FP32 drift, still small over 40 steps
+    c.close()
+
+
+def test_mixed_rejects_unsupported(jw):
+    c = jw.Context(32)
+    U = c.field()
+    with pytest.raises(jw.JwError) as e:
+        c.evolve_mixed(U, 3, 1, 1e-4, None, 1)
+    assert e.value.code == -4
+    c.close()
+    c = jw.Context(64)
+    U = c.field()
+    with pytest.raises(jw.JwError):
+        c.evolve_mixed(U, 3, 0, 1e-4, None, 1)     # LINEAR kernel
+    with pytest.raises(jw.JwError):
+        c.evolve_mixed(U, 2, 1, 1e-4, None, 1)     # HATTA
+    c.close()
