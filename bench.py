@@ -1,19 +1,30 @@
-"""Benchmark: Langevin steps/s of the MI355X JIMWLK solver on one GPU.
+"""Benchmark: Langevin steps/s of the MI355X JIMWLK solver on one GPU, mixed
+precision form against the plain FP64 form, with the dipole S(r) error of
+the mixed form against FP64.
 
 Workload (the configuration the headline metric is quoted on): N = 2048 periodic
-lattice, SU(3) Wilson lines in FP64, MV initial condition (Ny = 50 sheets,
-g^2 mu a = 0.25, m a = 1e-4; Philox charges), then momentum-space Langevin steps
-with running coupling and per-mode correlated noise (alpha frozen at
+lattice of SU(3) Wilson lines, MV initial condition (Ny = 50 sheets,
+g^2 mu a = 0.25, m a = 1e-4; Philox charges), then momentum-space Langevin
+steps with running coupling and per-mode correlated noise (alpha frozen at
 alpha_max = 0.7 in the IR, Lambda a = 0.02), SIN kernel, Delta s = 5e-5.
-Synthetic data: all noise comes from the Philox stream (seed 1).
+Synthetic data: all noise comes from the Philox stream.
+
+Forms, each timed over the same K steps from the same start:
+  * fp64  -- jw_evolve: FP64 throughout (the oracle-parity path);
+  * mixed -- jw_evolve_mixed: FP32 fields and FFTs, FP64 exponentials, U
+             projected onto SU(3) every 10 steps.
+`value` is the mixed form's steps/s; the FP64 form's steps/s, the ratio and
+S(r) after --sr-steps steps of both forms from the same MV start and Philox
+stream (max |S_mixed(r) - S_fp64(r)| over all lattice separations r) sit beside it.
 
 One step = noise generation, 2-D FFTs, kernel multiplies, U xi U^dagger and
 the SU(3) update; measurement is outside the timed region, as the reference's
 "Evolution time" excludes it (main_optimized.cpp:236-311).
 
-Prints one JSON line (driver contract) with `roofline` for the dominant kernel
-(HIP-event timing on the library's stream) and `cpu_baseline` (the CPU oracle
-timed on a bounded sample of the same workload on this host).
+Also printed: `roofline` for the dominant kernel of the mixed form (HIP-event
+timing on the library's stream; `roofline_fp64` likewise), per-kernel tables,
+and `cpu_baseline` (the CPU oracle timed on one step of the same workload on
+this host).
 """
 import argparse
 import json
@@ -30,9 +41,22 @@ METRIC = "Langevin steps/s at N=2048 SU(3): mixed+batched vs FP64; S(r) err vs F
 HBM_PEAK_GBS = 8000.0
 ALPHA_MAX = 0.7
 LAMBDA_A = 0.02
-# compulsory HBM bytes per site per step of the FP64 form (reals x 8 B):
+# compulsory HBM reals per site per step (x 8 B in FP64, x 4 B in FP32):
 # 182 for per-mode correlated noise, 132 for the white-noise couplings
 REALS_PER_SITE = {"correlated": 182, "white": 132}
+# Algorithmic HBM traffic of each kernel class of the fused step, in complex
+# planes per site per step (read + written; U is 9 planes):
+#   row_noise_fwd  W 8                                        8
+#   col_kmult      A: R 8 W 4 (+W 8 correlated), B: R 8 W 4   24 (32)
+#   row_uxiu_fwd   R U 9 (+ xi~ 8 correlated), W 8            17 (25)
+#   row_update     R A 4 + B 4 + U 9, W U 9                   26
+# Correlated: 91 planes = 182 reals per site, the compulsory figure.  White
+# noise: 75 planes = 150 reals (U is read by both row_uxiu_fwd and row_update).
+PLANES = {"correlated": {"row_noise_fwd": 8, "col_kmult": 32, "row_uxiu_fwd": 25, "row_update": 26},
+          "white": {"row_noise_fwd": 8, "col_kmult": 24, "row_uxiu_fwd": 17, "row_update": 26}}
+# measured HBM traffic per launch of the dominant kernels (rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a plain copy)
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_N2048.json")
 
 
 def parse():
@@ -42,16 +66,39 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--N", type=int, default=2048)
     p.add_argument("--coupling", default="correlated", choices=["correlated", "fixed", "noise", "sqrt"])
+    p.add_argument("--sr-steps", type=int, default=1000,
+                   help="steps of both forms before comparing S(r) (0: skip)")
+    p.add_argument("--reunit-every", type=int, default=10)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--calibrate", action="store_true",
                    help="also run a plain copy of 8 planes (k_calibration_copy) to scale traffic counters")
     p.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
-    p.add_argument("--form", default="fp64", choices=["fp64", "mixed"],
-                   help="plain FP64 form, or the mixed-precision form (FP32 fields and FFTs)")
-    p.add_argument("--reunit-every", type=int, default=10)
     p.add_argument("--fused-mode", type=int, default=1,
                    help="jw_set_fused mode: 1 default, 2/3/4 row-major/4-row/8-row blocked intermediates")
+    p.add_argument("--forms", default="fp64,mixed", help="forms to time (profiling runs may pick one)")
     return p.parse_args()
+
+
+def kernel_table(kt, nk, V, planes, bytes_per_real):
+    """Per-class time per step, algorithmic GB/s, and the dominant class's roofline."""
+    per_step_ms = {k: ms * n / nk for k, (ms, n) in kt.items() if n}
+    classes = {}
+    for k, t in per_step_ms.items():
+        if k in planes:
+            b = planes[k] * 2 * bytes_per_real * V
+            classes[k] = {"ms_per_step": round(t, 5), "GBs": round(b / (t * 1e-3) / 1e9, 1),
+                          "launch_ms": round(kt[k][0], 5), "launches_per_step": kt[k][1] / nk,
+                          "alg_bytes_per_step": b}
+    dom = max(classes, key=lambda k: classes[k]["ms_per_step"]) if classes else None
+    roof = None
+    if dom:
+        dom_ms, dom_calls = kt[dom]
+        bytes_per_launch = classes[dom]["alg_bytes_per_step"] * nk / dom_calls
+        ach = bytes_per_launch / (dom_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(dom_ms, 5),
+                "alg_bytes_per_launch": bytes_per_launch}
+    return classes, roof
 
 
 def main():
@@ -64,13 +111,14 @@ def main():
     import numpy as np
     import jimwlk as jw
 
-    N = a.N
+    N, V = a.N, a.N * a.N
+    forms = [f for f in a.forms.split(",") if f]
     coupling = {"correlated": jw.CORRELATED_NOISE_COUPLING, "fixed": jw.FIXED_COUPLING,
                 "noise": jw.NOISE_COUPLING_CONSTANT, "sqrt": jw.SQRT_COUPLING_CONSTANT}[a.coupling]
     step = 5e-5 if a.coupling == "correlated" else 1e-4
-    al = jw.alpha(lambda_L=LAMBDA_A * N, mu0_over_lambda=math.exp(2 * math.pi / (9 * ALPHA_MAX)))
-    if a.coupling != "correlated":
-        al = None
+    al = jw.alpha(lambda_L=LAMBDA_A * N, mu0_over_lambda=math.exp(2 * math.pi / (9 * ALPHA_MAX))) \
+        if a.coupling in ("correlated", "noise", "sqrt") else None
+    kind = "correlated" if a.coupling == "correlated" else "white"
 
     ctx = jw.Context(N, 0)
     ctx.set_fused(a.fused_mode)
@@ -80,65 +128,78 @@ def main():
     t0 = time.perf_counter()
     ctx.mv_init(U, 50, 1.0, g2mu, 1e-4 / g2mu)
     ctx.sync()
-    t_mv = time.perf_counter() - t0
-    print(f"bench: N={N} MV init (50 sheets) {t_mv * 1e3:.1f} ms", file=sys.stderr, flush=True)
-
+    print(f"bench: N={N} MV init (50 sheets) {(time.perf_counter() - t0) * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    U0 = U.download()
     if a.calibrate:
         ctx.calibration_copy(8)
         ctx.sync()
-    def run(n):
-        if a.form == "mixed":
-            ctx.evolve_mixed(U, coupling, jw.SIN_KERNEL, step, al, nsteps=n, reunit_every=a.reunit_every)
+
+    def run(form, F, n):
+        if form == "mixed":
+            ctx.evolve_mixed(F, coupling, jw.SIN_KERNEL, step, al, nsteps=n, reunit_every=a.reunit_every)
         else:
-            ctx.evolve(U, coupling, jw.SIN_KERNEL, step, al, nsteps=n)
+            ctx.evolve(F, coupling, jw.SIN_KERNEL, step, al, nsteps=n)
 
-    run(a.warmup)
-    ctx.sync()
-    t0 = time.perf_counter()
-    run(a.steps)
-    ctx.sync()
-    dt = time.perf_counter() - t0
-    sps = a.steps / dt
-    print(f"bench: {a.steps} steps in {dt:.3f} s -> {sps:.1f} steps/s", file=sys.stderr, flush=True)
+    res = {}
+    for form in forms:
+        F = ctx.field(9, U0)
+        ctx.philox(2, 0, 0)
+        run(form, F, a.warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        run(form, F, a.steps)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        sps = a.steps / dt
+        print(f"bench: {form}: {a.steps} steps in {dt:.4f} s -> {sps:.1f} steps/s", file=sys.stderr, flush=True)
+        # per-kernel durations with HIP events on the library stream (separate pass)
+        nk = min(a.steps, 10)
+        ctx.timing(True)
+        run(form, F, nk)
+        kt = ctx.kernel_times()
+        ctx.timing(False)
+        bpr = 4 if form == "mixed" else 8
+        classes, roof = kernel_table(kt, nk, V, PLANES[kind], bpr)
+        print(f"bench: {form} kernel classes {json.dumps({k: v['ms_per_step'] for k, v in classes.items()})}",
+              file=sys.stderr, flush=True)
+        step_bytes = REALS_PER_SITE[kind] * bpr * V
+        res[form] = {"steps_per_s": round(sps, 3), "ms_per_step": round(1e3 / sps, 4), "kernels": classes,
+                     "roofline": roof,
+                     "step_roofline": {"bytes_per_step": step_bytes,
+                                       "achieved_GBs": round(step_bytes * sps / 1e9, 1),
+                                       "frac": round(step_bytes * sps / (HBM_PEAK_GBS * 1e9), 4),
+                                       "reals_per_site": REALS_PER_SITE[kind], "bytes_per_real": bpr}}
+        F.close()
 
-    # per-kernel durations with HIP events on the library stream (separate pass)
-    ctx.timing(True)
-    run(min(a.steps, 10))
-    kt = ctx.kernel_times()
-    ctx.timing(False)
-    # Algorithmic HBM bytes per site per step of each kernel class of the fused
-    # step (complex planes of 16 B each way; U is 9 complex planes):
-    #   row_noise_fwd  W 8 planes                                  128
-    #   col_kmult      A: R 8 W 4 (+W 8 correlated), B: R 8 W 4    384 (512)
-    #   row_uxiu_fwd   R U 9 (+ xi~ 8 correlated), W 8             272 (400)
-    #   row_update     R A 4 + B 4 + U 9, W U 9                    416
-    # Correlated: 1456 B = 182 reals per site, the compulsory figure.  White
-    # noise: 1200 B = 150 reals (U is read by both row_uxiu_fwd and row_update).
-    kind = "correlated" if a.coupling == "correlated" else "white"
-    corr_ = kind == "correlated"
-    alg = {"row_noise_fwd": 128, "col_kmult": 512 if corr_ else 384, "row_uxiu_fwd": 400 if corr_ else 272,
-           "row_update": 416}
-    V = N * N
-    nk = min(a.steps, 10)
-    per_step_ms = {k: ms * n / nk for k, (ms, n) in kt.items()}
-    dom = max(per_step_ms, key=per_step_ms.get)
-    classes = {}
-    for k, t in per_step_ms.items():
-        if k in alg:
-            classes[k] = {"ms_per_step": round(t, 5), "GBs": round(alg[k] * V / (t * 1e-3) / 1e9, 1),
-                          "launch_ms": round(kt[k][0], 5), "launches_per_step": kt[k][1] / nk}
-    roof = None
-    if dom in alg:
-        dom_ms, dom_calls = kt[dom]
-        bytes_per_launch = alg[dom] * V * nk / dom_calls
-        ach = bytes_per_launch / (dom_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(dom_ms, 5),
-                "alg_bytes_per_launch": bytes_per_launch}
-    step_bytes = REALS_PER_SITE[kind] * 8.0 * V
-    step_frac = step_bytes * sps / (HBM_PEAK_GBS * 1e9)
-    print(f"bench: kernel classes {json.dumps({k: [round(v[0], 4), v[1]] for k, v in kt.items()})}",
-          file=sys.stderr, flush=True)
+    # measured traffic of the dominant kernels, when a profile summary is present
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f)
+        for form, r in res.items():
+            roof = r["roofline"]
+            key = f"{form}:{a.coupling}:{roof['kernel']}" if roof else None
+            if key in traffic:
+                roof["traffic"] = traffic[key]["bytes_per_launch"]
+                roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, REPO)
+
+    # S(r) of the mixed form against FP64 after the same sr-steps from the same start
+    sr = None
+    if a.sr_steps > 0 and "fp64" in forms and "mixed" in forms:
+        S = {}
+        for form in ("fp64", "mixed"):
+            F = ctx.field(9, U0)
+            ctx.philox(3, 0, 0)
+            run(form, F, a.sr_steps)
+            S[form] = ctx.dipole_from_corr(ctx.measure_corr(F))
+            F.close()
+        S0 = ctx.dipole_from_corr(ctx.measure_corr(ctx.field(9, U0)))
+        r, s64, cnt = jw.radial_bins(S["fp64"], N, N // 2)
+        _, s32, _ = jw.radial_bins(S["mixed"], N, N // 2)
+        sr = {"steps": a.sr_steps, "max_abs_err": float(np.abs(S["fp64"] - S["mixed"]).max()),
+              "max_abs_err_radial": float(np.abs(s64 - s32)[cnt > 0].max()),
+              "max_abs_change_from_start": float(np.abs(S["fp64"] - S0).max()), "bound": 2e-5}
+        print(f"bench: S(r) after {a.sr_steps} steps: max |mixed - fp64| = {sr['max_abs_err']:.3e} "
+              f"(evolution moved S by {sr['max_abs_change_from_start']:.3e})", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -147,12 +208,10 @@ def main():
         if a.cpu_threads > 0:
             orc.set_num_threads(a.cpu_threads)
         nthr = orc.num_threads()
-        import numpy as np
         rng = np.random.default_rng(0)
-        U0 = U.download()
         z = rng.normal(size=V * 16)
         oal = oracle.alpha(lambda_L=LAMBDA_A * N, mu0_over_lambda=math.exp(2 * math.pi / (9 * ALPHA_MAX))) \
-            if a.coupling == "correlated" else None
+            if al is not None else None
         oc = {"correlated": oracle.CORRELATED, "fixed": oracle.FIXED, "noise": oracle.NOISE,
               "sqrt": oracle.SQRT}[a.coupling]
         t0 = time.perf_counter()
@@ -160,20 +219,32 @@ def main():
         tc = time.perf_counter() - t0
         cpu = {"value": round(1.0 / tc, 5), "unit": "steps/s", "cores": nthr, "kind": "port",
                "sample": f"1 Langevin step at N={N} ({a.coupling} coupling) by the CPU oracle "
-                         f"(reference algorithm: 9-component FFTs, eigen-decomposition exponential), "
+                         f"(reference algorithm in FP64: 9-component FFTs, eigen-decomposition exponential), "
                          f"{tc:.1f} s with {nthr} OpenMP threads"}
         print(f"bench: cpu oracle step {tc:.2f} s ({nthr} threads)", file=sys.stderr, flush=True)
 
-    out = {"metric": METRIC, "value": round(sps, 3), "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
-           "warmup": a.warmup, "ms_per_step": round(1e3 / sps, 4), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64", "data": "synthetic (Philox noise, MV initial condition)",
-           "config": {"workload": f"N={N} FP64 Langevin step, {a.coupling} coupling"
+    head = "mixed" if "mixed" in res else forms[0]
+    h = res[head]
+    out = {"metric": METRIC, "value": h["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32" if head == "mixed" else "f64",
+           "data": "synthetic (Philox noise, MV initial condition)",
+           "config": {"workload": f"N={N} Langevin step, {a.coupling} coupling"
                                   + (f" (alpha_max={ALPHA_MAX}, Lambda a={LAMBDA_A})" if a.coupling == "correlated" else "")
-                                  + f", SIN kernel, ds={step}", "N": N, "sites": V,
-                      "form": "plain FP64", "parallelism": "single GPU"},
-           "step_roofline": {"bytes_per_step": step_bytes, "achieved_GBs": round(step_bytes * sps / 1e9, 1),
-                             "frac": round(step_frac, 4), "reals_per_site": REALS_PER_SITE[kind]},
-           "roofline": roof, "kernels": classes, "cpu_baseline": cpu}
+                                  + f", SIN kernel, ds={step}, MV start Ny=50 g2mu a=0.25 m a=1e-4",
+                      "N": N, "sites": V,
+                      "form": ("mixed: FP32 fields and FFTs, FP64 exponentials, SU(3) projection every "
+                               f"{a.reunit_every} steps; one trajectory") if head == "mixed" else "plain FP64",
+                      "parallelism": "single GPU"},
+           "fp64_steps_per_s": res["fp64"]["steps_per_s"] if "fp64" in res else None,
+           "mixed_over_fp64": round(res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"], 3)
+           if "mixed" in res and "fp64" in res else None,
+           "sr_err_vs_fp64": sr,
+           "roofline": h["roofline"], "step_roofline": h["step_roofline"], "kernels": h["kernels"],
+           "roofline_fp64": res["fp64"]["roofline"] if "fp64" in res else None,
+           "step_roofline_fp64": res["fp64"]["step_roofline"] if "fp64" in res else None,
+           "kernels_fp64": res["fp64"]["kernels"] if "fp64" in res else None,
+           "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -43,14 +43,15 @@ struct RowE {
     static_assert(SPT >= 1 && SPT * NT == N, "row pass needs N a multiple of its thread count");
 };
 // Row-major column passes hold the X and Y tiles of one W-column strip together
-// in LDS (2 W N complex values plus one padding row in eight, 144 KB at most):
-// W = min(16, 4096 / N) for FP64, twice that for FP32.
-template <int N, typename C = double2>
+// in LDS (2 W N complex values plus one padding row in eight, 144 KB at most in
+// FP64): W = min(16, 4096 / N) for both forms (wider FP32 strips measured
+// slower: one workgroup per CU and spills in the correlated variant).
+template <int N, typename C = double2, int CORR = 0>
 struct ColE {
-    static constexpr int W0 = 4096 * (int)(sizeof(double2) / sizeof(C)) / N;    // 128 KB of tiles
-    static constexpr int W = W0 > 16 ? 16 : W0;
+    static constexpr int W = (4096 / N > 16) ? 16 : 4096 / N;
     static constexpr int ELEMS = 2 * W * N;
-    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 1024) ? 1024 : ELEMS / 8);
+    static constexpr int NTMAX = 1024;
+    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > NTMAX) ? NTMAX : ELEMS / 8);
     static constexpr int PER = ELEMS / NT;          // elements per thread
     static constexpr int PAIRS = (W * N) / NT;      // (X, Y) element pairs per thread
     static_assert(W >= 1 && W <= N && N % W == 0 && PER * NT == ELEMS && PAIRS * NT == W * N, "column tiling");
@@ -209,8 +210,9 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(C* __restrict__ Vv
 // row pass 2, correlated noise: reads the column-inverse spectra of xi~,
 // finishes their inverse transform along the row, and forms V_i = U xi~_i U^dagger
 // in two batches (i = x, then y).  In each batch a site's V coefficients
-// overwrite its own xi~ slots in LDS before the forward row FFT; U is read once
-// per batch (the second read is served by the caches).
+// overwrite its own xi~ slots in LDS before the forward row FFT.  U is read in
+// both batches: holding xi~_x in registers to read U once needs more registers
+// and measured slower (one workgroup per CU instead of two in FP32).
 // ---------------------------------------------------------------------------
 template <int N, int BF, typename C>
 __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_corr(C* __restrict__ Vv, const C* __restrict__ U,
@@ -272,14 +274,15 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
 // with element stride 2W.
 // ---------------------------------------------------------------------------
 template <int N, int CORR, typename C>
-__global__ __launch_bounds__((ColE<N, C>::NT)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
+__global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
                                                           C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                           const double* __restrict__ q4,
                                                           const double* __restrict__ sb,
                                                           const double* __restrict__ sa,
                                                           const C* __restrict__ tw) {
     typedef typename scalar_of<C>::type T;
-    constexpr int W = ColE<N, C>::W, NT = ColE<N, C>::NT, PER = ColE<N, C>::PER, PAIRS = ColE<N, C>::PAIRS;
+    typedef ColE<N, C, CORR> CE;
+    constexpr int W = CE::W, NT = CE::NT, PER = CE::PER, PAIRS = CE::PAIRS;
     constexpr int S2 = 2 * W;
     __shared__ C lds[padded<N, 3>::value * S2];
     const int tid = threadIdx.x;
@@ -296,7 +299,7 @@ __global__ __launch_bounds__((ColE<N, C>::NT)) void k_col_kmult(C* __restrict__ 
         lds[lpos<S2, 1, 3>(x, s)] = s < W ? X[g] : Y[g];
     }
     __syncthreads();
-    lds_fft<N, S2, S2, 1, NT, -1, 3>(lds, tw, tid);
+    lds_fft<N, S2, S2, 1, NT, -1, 3>(lds, opaque(tw), opaque_v(tid));
     // pair (x, c): X^ in slot c of LDS row x, Y^ in slot W + c
     C zreg[CORR ? PAIRS : 1];
 #pragma unroll
@@ -326,7 +329,7 @@ __global__ __launch_bounds__((ColE<N, C>::NT)) void k_col_kmult(C* __restrict__ 
     __syncthreads();
     if (CORR) {
         // xi~ = F_col^-1 of the scaled tiles, written back over X and Y
-        lds_fft<N, S2, S2, 1, NT, +1, 3>(lds, tw, tid);
+        lds_fft<N, S2, S2, 1, NT, +1, 3>(lds, opaque(tw), opaque_v(tid));
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = tid + q * NT;
@@ -344,7 +347,7 @@ __global__ __launch_bounds__((ColE<N, C>::NT)) void k_col_kmult(C* __restrict__ 
         __syncthreads();
     }
     // Z: the W sequences in the X slots
-    lds_fft<N, W, S2, 1, NT, +1, 3>(lds, tw, tid);
+    lds_fft<N, W, S2, 1, NT, +1, 3>(lds, opaque(tw), opaque_v(tid));
 #pragma unroll
     for (int q = 0; q < PAIRS; q++) {
         const int pr = tid + q * NT;
@@ -549,12 +552,13 @@ static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int 
     const size_t V = c->V;
     const C* tw = twiddles<C>(c);
     if constexpr (BF == 0) {
-        constexpr int W = ColE<N, C>::W, NT = ColE<N, C>::NT;
+        constexpr int W = ColE<N, C>::W;
         if (corr)
-            hipLaunchKernelGGL((k_col_kmult<N, 1, C>), dim3(N / W, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
+            hipLaunchKernelGGL((k_col_kmult<N, 1, C>), dim3(N / W, 4), dim3(ColE<N, C, 1>::NT), 0, c->stream, Z, X,
+                               Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
         else
-            hipLaunchKernelGGL((k_col_kmult<N, 0, C>), dim3(N / W, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
+            hipLaunchKernelGGL((k_col_kmult<N, 0, C>), dim3(N / W, 4), dim3(ColE<N, C, 0>::NT), 0, c->stream, Z, X,
+                               Y, V,
                                sqrt_coupling, c->q4, c->sb, sa_k, tw);
     } else {
         constexpr int CPB = ColB<N, BF, C>::CPB, NT = ColB<N, BF, C>::NT;
