@@ -152,6 +152,37 @@ int jw_evolve(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, c
 int jw_evolve_mixed(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
                     int nsteps, int reunit_every);
 
+/* ---- batched form ------------------------------------------------------
+ * No counterpart in the reference (one lattice per run); it advances many
+ * independent trajectories -- the reference's `stat` loop of main_optimized.cpp
+ * and its MV + evolution body, run concurrently -- as size groups of equal N
+ * (64 <= N <= 2048): group i holds count[i] trajectories of N[i]^2, numbered
+ * consecutively over the groups.  fp32 = 1: the mixed-precision form (as
+ * jw_evolve_mixed), 0: FP64 (as jw_evolve).  Each group's kernels take all its
+ * trajectories in one launch; one step of all groups (their launches forked
+ * onto one stream per group) is captured once into a HIP graph and replayed.
+ * Trajectory t draws the Philox stream of key {seed, traj0 + t}, so it is the
+ * trajectory a single lattice evolves after jw_noise_philox(seed, traj0 + t,
+ * step).  The intermediate-plane layout of each group is chosen on first use
+ * by timing all three on a copy of its state (jw_batch_set_layout overrides). */
+typedef struct jw_batch jw_batch;
+int jw_batch_create(jw_batch** out, int device, int ngroups, const int* N, const int* count, int fp32);
+void jw_batch_destroy(jw_batch* b);
+int jw_batch_size(const jw_batch* b);
+/* copy trajectory t from / to a 9-component field of a context with the same N */
+int jw_batch_set(jw_batch* b, int t, const jw_field* U);
+int jw_batch_get(jw_batch* b, int t, jw_field* U);
+/* noise key and step of every trajectory (trajectory t: key {seed, traj0 + t}) */
+int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step);
+/* layout 0, 4 or 8 for group g, or -1 to tune on the next evolution; the chosen one */
+int jw_batch_set_layout(jw_batch* b, int group, int layout);
+int jw_batch_layout(const jw_batch* b, int group);
+/* nsteps steps of every trajectory (SIN kernel); reunit_every as jw_evolve_mixed
+ * (ignored in FP64).  Asynchronous: jw_batch_sync waits. */
+int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw_alpha* alpha, int nsteps,
+                    int reunit_every);
+int jw_batch_sync(jw_batch* b);
+
 /* MV initial condition (main_optimized.cpp:178-191): U = 1, then Ny sheets
  * U <- U exp(i A_s), A_s = F^-1[ F rho_s * (-g / (-phat2 + m^2)) ], rho_s with
  * sigma = g mu / sqrt(Ny), m = mass * g^2 * mu.  Charges: injected (8*V*Ny

@@ -117,7 +117,7 @@ int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, jw::c
         jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
         if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
         if (fused) {
-            const jw::StepArgs sa{ns, sa_pos, sa_k, kc == 0, corr, ss, 0};
+            const jw::StepArgs sa{ns, sa_pos, sa_k, kc == 0, corr, ss, 0, 1};
             HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, sa));
             continue;
         }
@@ -131,6 +131,175 @@ int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, jw::c
         HIP_CHECK(jw::kernel_mult_packed(c, ZB, Vv, Vv + 4 * V, kc, sa_k, 0));
         HIP_CHECK(jw::fft2d(c, ZB, 4, 0, s));
         HIP_CHECK(jw::update_packed(c, U->d, ZA, ZB, ss));
+    }
+    return JW_OK;
+}
+
+
+// ---------------------------------------------------------------- batches
+// One size group of a batch: `count` trajectories of one N, state arrays of
+// `count` x (9, 8, 8, 4, 4) planes (U, xi, V, A, B), element type by precision.
+struct BatchGroup {
+    jw_ctx* ctx = nullptr;     // tables, coupling table and stream for this N
+    int N = 0, count = 0, first = 0;
+    void* U = nullptr;
+    void* Xi = nullptr;
+    void* Vv = nullptr;
+    void* ZA = nullptr;
+    void* ZB = nullptr;
+    int layout = -1;           // -1: tune on first use
+    hipEvent_t join = nullptr;
+};
+
+// key of the captured step graph: the arguments baked into it
+struct BatchKey {
+    int coupling, reunit_every;
+    double step;
+    jw::Alpha alpha;
+    bool operator==(const BatchKey& o) const {
+        return coupling == o.coupling && reunit_every == o.reunit_every && step == o.step &&
+               memcmp(&alpha, &o.alpha, sizeof alpha) == 0;
+    }
+};
+
+}  // namespace
+
+struct jw_batch {
+    int device = 0, fp32 = 0, ntraj = 0;
+    std::vector<BatchGroup> g;
+    hipStream_t stream = nullptr;    // origin of the step graph; groups fork from it
+    hipEvent_t fork = nullptr;
+    uint32_t* d_step = nullptr;      // Philox step counter on the device
+    uint32_t seed = 0, traj0 = 0, step = 0;
+    hipGraphExec_t graph = nullptr;  // reunit_every steps (10 without projection)
+    int graph_steps = 0;
+    bool have_key = false;
+    BatchKey key{};
+};
+
+namespace {
+
+size_t elem_size(const jw_batch* b) { return b->fp32 ? sizeof(jw::cplxf) : sizeof(cplx); }
+
+void batch_free(jw_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    for (auto& g : b->g) {
+        for (void* p : {g.U, g.Xi, g.Vv, g.ZA, g.ZB})
+            if (p) hipFree(p);
+        if (g.join) hipEventDestroy(g.join);
+        if (g.ctx) jw_destroy(g.ctx);
+    }
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->d_step) hipFree(b->d_step);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// locate trajectory t: its group and index inside the group
+BatchGroup* batch_find(jw_batch* b, int t, int* k) {
+    for (auto& g : b->g)
+        if (t >= g.first && t < g.first + g.count) {
+            *k = t - g.first;
+            return &g;
+        }
+    return nullptr;
+}
+
+// coupling tables of one group for this evolution
+int batch_tables(BatchGroup& g, int coupling, const jw::Alpha& al, const double** sa_pos, const double** sa_k) {
+    jw_ctx* c = g.ctx;
+    *sa_pos = *sa_k = nullptr;
+    if (coupling == JW_NOISE_COUPLING_CONSTANT) {
+        if (int e = ensure_sa(c, 1, al)) return e;
+        *sa_pos = c->sa;
+    } else if (coupling == JW_SQRT_COUPLING_CONSTANT || coupling == JW_CORRELATED_NOISE_COUPLING) {
+        if (int e = ensure_sa(c, 0, al)) return e;
+        if (c->layout != 0) {
+            if (int e = ensure_sa16(c)) return e;
+            *sa_k = c->sa16;
+        } else {
+            *sa_k = c->sa;
+        }
+    }
+    return JW_OK;
+}
+
+// one step of group g on its own stream, noise step from the device counter
+hipError_t batch_group_step(jw_batch* b, BatchGroup& g, int coupling, double step, const double* sa_pos,
+                            const double* sa_k, int reunit, int count, void* U) {
+    jw::NoiseSrc ns{nullptr, b->seed, b->traj0 + (uint32_t)g.first, 0u, JW_TAG_LANGEVIN, b->d_step};
+    const jw::StepArgs a{ns, sa_pos, sa_k, coupling == JW_SQRT_COUPLING_CONSTANT,
+                         coupling == JW_CORRELATED_NOISE_COUPLING, sqrt(step), reunit, count};
+    if (b->fp32)
+        return jw::step_fused_f32(g.ctx, (jw::cplxf*)U, (jw::cplxf*)g.Xi, (jw::cplxf*)g.Vv, (jw::cplxf*)g.ZA,
+                                  (jw::cplxf*)g.ZB, a);
+    return jw::step_fused(g.ctx, (cplx*)U, (cplx*)g.Xi, (cplx*)g.Vv, (cplx*)g.ZA, (cplx*)g.ZB, a);
+}
+
+// one step of every group, forked from and joined back to b->stream, then the
+// step counter; capturable
+int batch_step_all(jw_batch* b, int coupling, double step, const jw::Alpha& al, int reunit) {
+    HIP_CHECK(hipEventRecord(b->fork, b->stream));
+    for (auto& g : b->g) {
+        const double *sa_pos, *sa_k;
+        if (int e = batch_tables(g, coupling, al, &sa_pos, &sa_k)) return e;
+        HIP_CHECK(hipStreamWaitEvent(g.ctx->stream, b->fork, 0));
+        HIP_CHECK(batch_group_step(b, g, coupling, step, sa_pos, sa_k, reunit, g.count, g.U));
+        HIP_CHECK(hipEventRecord(g.join, g.ctx->stream));
+        HIP_CHECK(hipStreamWaitEvent(b->stream, g.join, 0));
+    }
+    HIP_CHECK(jw::step_inc(b->stream, b->d_step));
+    return JW_OK;
+}
+
+// The tuner: for each group with no layout yet, time 3 steps (after one
+// untimed) of each intermediate layout on a copy of its U with a scratch noise
+// key, and keep the fastest.  The trajectories and the step counter are left
+// as they were.
+int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
+    for (auto& g : b->g) {
+        if (g.layout >= 0) continue;
+        const size_t ubytes = (size_t)g.count * 9 * g.ctx->V * elem_size(b);
+        void* scratch = nullptr;
+        if (hipMalloc(&scratch, ubytes) != hipSuccess) return fail(JW_ERR_ALLOC, "tuning buffer allocation failed");
+        hipMemcpyAsync(scratch, g.U, ubytes, hipMemcpyDeviceToDevice, g.ctx->stream);
+        const uint32_t saved_traj0 = b->traj0;
+        b->traj0 = 0xFFFFF000u;      // scratch noise key
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        float best = 1e30f;
+        int best_layout = 0;
+        int rc = JW_OK;
+        for (int lay : {0, 4, 8}) {
+            g.ctx->layout = lay;
+            const double *sa_pos, *sa_k;
+            if ((rc = batch_tables(g, coupling, al, &sa_pos, &sa_k))) break;
+            float ms = 0.f;
+            for (int it = 0; it < 4 && rc == JW_OK; it++) {
+                if (it == 1) hipEventRecord(e0, g.ctx->stream);
+                if (batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, scratch) != hipSuccess)
+                    rc = fail(JW_ERR_HIP, "tuning step failed");
+            }
+            hipEventRecord(e1, g.ctx->stream);
+            hipEventSynchronize(e1);
+            hipEventElapsedTime(&ms, e0, e1);
+            if (rc == JW_OK && ms < best) {
+                best = ms;
+                best_layout = lay;
+            }
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        b->traj0 = saved_traj0;
+        hipStreamSynchronize(g.ctx->stream);
+        hipFree(scratch);
+        if (rc) return rc;
+        g.layout = best_layout;
+        g.ctx->layout = best_layout;
     }
     return JW_OK;
 }
@@ -545,7 +714,7 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
         }
         // projection every reunit_every steps and after the last one
         const int reunit = reunit_every > 0 && ((it + 1) % reunit_every == 0 || it == nsteps - 1);
-        const jw::StepArgs sa{ns, sa_pos, sa_k, coupling == JW_SQRT_COUPLING_CONSTANT, corr, sqrt(step), reunit};
+        const jw::StepArgs sa{ns, sa_pos, sa_k, coupling == JW_SQRT_COUPLING_CONSTANT, corr, sqrt(step), reunit, 1};
         const hipError_t he = jw::step_fused_f32(c, U32, Xi, Vv, ZA, ZB, sa);
         if (he != hipSuccess) rc = fail(JW_ERR_HIP, "%s", hipGetErrorString(he));
     }
@@ -710,6 +879,198 @@ int jw_su3_exp_gen(jw_ctx* c, const double* h8, double s, double* out18, int n) 
     hipFree(dh);
     hipFree(dout);
     if (e != hipSuccess) return fail(JW_ERR_HIP, "exp kernel: %s", hipGetErrorString(e));
+    return JW_OK;
+}
+
+
+// ---------------------------------------------------------------- batches
+int jw_batch_create(jw_batch** out, int device, int ngroups, const int* N, const int* count, int fp32) {
+    if (!out || ngroups < 1 || ngroups > 64 || !N || !count) return fail(JW_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    for (int i = 0; i < ngroups; i++) {
+        if (!valid_N(N[i]) || !jw::fused_supported(N[i]))
+            return fail(JW_ERR_UNSUPPORTED, "batch group %d: N = %d (batches need 64 <= N <= 2048)", i, N[i]);
+        if (count[i] < 1) return fail(JW_ERR_ARG, "batch group %d: count %d", i, count[i]);
+    }
+    int ndev = 0;
+    HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(JW_ERR_ARG, "device %d of %d", device, ndev);
+    HIP_CHECK(hipSetDevice(device));
+    jw_batch* b = new jw_batch();
+    b->device = device;
+    b->fp32 = fp32 ? 1 : 0;
+    b->g.resize(ngroups);
+    int rc = JW_OK;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipMalloc(&b->d_step, sizeof(uint32_t)) != hipSuccess)
+        rc = fail(JW_ERR_HIP, "batch stream/event/counter creation failed");
+    const size_t es = elem_size(b);
+    for (int i = 0; rc == JW_OK && i < ngroups; i++) {
+        BatchGroup& g = b->g[i];
+        g.N = N[i];
+        g.count = count[i];
+        g.first = b->ntraj;
+        b->ntraj += count[i];
+        if ((rc = jw_create(&g.ctx, N[i], device))) break;
+        g.ctx->layout = -1;
+        const size_t V = g.ctx->V, n = (size_t)g.count * V * es;
+        if (hipMalloc(&g.U, 9 * n) != hipSuccess || hipMalloc(&g.Xi, 8 * n) != hipSuccess ||
+            hipMalloc(&g.Vv, 8 * n) != hipSuccess || hipMalloc(&g.ZA, 4 * n) != hipSuccess ||
+            hipMalloc(&g.ZB, 4 * n) != hipSuccess) {
+            rc = fail(JW_ERR_ALLOC, "batch group %d: %zu bytes of state", i, 33 * n);
+            break;
+        }
+        if (hipEventCreateWithFlags(&g.join, hipEventDisableTiming) != hipSuccess) rc = fail(JW_ERR_HIP, "event");
+    }
+    if (rc == JW_OK) rc = jw_batch_philox(b, 0, 0, 0);
+    if (rc) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return JW_OK;
+}
+
+void jw_batch_destroy(jw_batch* b) { batch_free(b); }
+
+int jw_batch_size(const jw_batch* b) { return b ? b->ntraj : 0; }
+
+int jw_batch_set(jw_batch* b, int t, const jw_field* U) {
+    if (!b || !U) return fail(JW_ERR_ARG, "null argument");
+    int k;
+    BatchGroup* g = batch_find(b, t, &k);
+    if (!g) return fail(JW_ERR_ARG, "trajectory %d of %d", t, b->ntraj);
+    if (U->t != 9 || U->ctx->N != g->N) return fail(JW_ERR_ARG, "field is not a 9-component %d^2 lattice", g->N);
+    const size_t V = g->ctx->V;
+    HIP_CHECK(hipStreamSynchronize(U->ctx->stream));     // pending work on the source field
+    HIP_CHECK(hipStreamSynchronize(b->stream));           // and on the batch
+    if (b->fp32) {
+        HIP_CHECK(jw::u_convert(g->ctx, (jw::cplxf*)g->U + (size_t)k * 9 * V, U->d, 1));
+    } else {
+        HIP_CHECK(hipMemcpyAsync((cplx*)g->U + (size_t)k * 9 * V, U->d, 9 * V * sizeof(cplx),
+                                 hipMemcpyDeviceToDevice, g->ctx->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(g->ctx->stream));
+    return JW_OK;
+}
+
+int jw_batch_get(jw_batch* b, int t, jw_field* U) {
+    if (!b || !U) return fail(JW_ERR_ARG, "null argument");
+    int k;
+    BatchGroup* g = batch_find(b, t, &k);
+    if (!g) return fail(JW_ERR_ARG, "trajectory %d of %d", t, b->ntraj);
+    if (U->t != 9 || U->ctx->N != g->N) return fail(JW_ERR_ARG, "field is not a 9-component %d^2 lattice", g->N);
+    const size_t V = g->ctx->V;
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    HIP_CHECK(hipStreamSynchronize(U->ctx->stream));     // pending work on the destination field
+    if (b->fp32) {
+        HIP_CHECK(jw::u_convert(g->ctx, (jw::cplxf*)g->U + (size_t)k * 9 * V, U->d, 0));
+    } else {
+        HIP_CHECK(hipMemcpyAsync(U->d, (cplx*)g->U + (size_t)k * 9 * V, 9 * V * sizeof(cplx),
+                                 hipMemcpyDeviceToDevice, g->ctx->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(g->ctx->stream));
+    return JW_OK;
+}
+
+int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step) {
+    if (!b) return fail(JW_ERR_ARG, "null batch");
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    b->seed = seed;
+    b->traj0 = traj0;
+    b->step = step;
+    HIP_CHECK(hipMemcpy(b->d_step, &step, sizeof step, hipMemcpyHostToDevice));
+    if (b->graph) {      // the key of the noise is baked into the graph
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->have_key = false;
+    }
+    return JW_OK;
+}
+
+int jw_batch_set_layout(jw_batch* b, int group, int layout) {
+    if (!b || group < 0 || group >= (int)b->g.size()) return fail(JW_ERR_ARG, "bad batch group");
+    if (layout != -1 && layout != 0 && layout != 4 && layout != 8) return fail(JW_ERR_ARG, "layout %d", layout);
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    b->g[group].layout = layout;
+    b->g[group].ctx->layout = layout;
+    if (b->graph) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->have_key = false;
+    }
+    return JW_OK;
+}
+
+int jw_batch_layout(const jw_batch* b, int group) {
+    if (!b || group < 0 || group >= (int)b->g.size()) return -2;
+    return b->g[group].layout;
+}
+
+int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw_alpha* a, int nsteps,
+                    int reunit_every) {
+    if (!b) return fail(JW_ERR_ARG, "null batch");
+    if (coupling == JW_HATTA_COUPLING_CONSTANT)
+        return fail(JW_ERR_UNSUPPORTED, "HATTA coupling: the reference leaves the coupling uninitialised here");
+    if (coupling < 0 || coupling > 4 || kernel != JW_SIN_KERNEL)
+        return fail(kernel == JW_LINEAR_KERNEL ? JW_ERR_UNSUPPORTED : JW_ERR_ARG, "batches run the SIN kernel");
+    if (!(step >= 0) || nsteps < 0 || reunit_every < 0) return fail(JW_ERR_ARG, "bad step/nsteps/reunit_every");
+    if (nsteps == 0) return JW_OK;
+    if (!b->fp32) reunit_every = 0;      // the FP64 form is not projected (as jw_evolve)
+    HIP_CHECK(hipSetDevice(b->device));
+    const jw::Alpha al = to_alpha(a);
+    if (int e = batch_tune(b, coupling, step, al)) return e;
+    // steps per graph: one projection period (10 without projection)
+    const int K = reunit_every > 0 ? reunit_every : 10;
+    const BatchKey key{coupling, reunit_every, step, al};
+    if (b->graph && !(b->have_key && key == b->key)) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    // every group's stream must be idle before the origin stream forks to it
+    for (auto& g : b->g) HIP_CHECK(hipStreamSynchronize(g.ctx->stream));
+    if (!b->graph && nsteps >= K) {
+        // coupling tables first: their builds are not captured
+        for (auto& g : b->g) {
+            const double *p, *k;
+            if (int e = batch_tables(g, coupling, al, &p, &k)) return e;
+            HIP_CHECK(hipStreamSynchronize(g.ctx->stream));
+        }
+        hipGraph_t graph = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = JW_OK;
+        for (int it = 0; it < K && rc == JW_OK; it++)
+            rc = batch_step_all(b, coupling, step, al, reunit_every > 0 && it == K - 1);
+        const hipError_t ec = hipStreamEndCapture(b->stream, &graph);
+        if (rc) {
+            if (graph) hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(JW_ERR_HIP, "step capture: %s", hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            b->graph = nullptr;
+            return fail(JW_ERR_HIP, "step graph: %s", hipGetErrorString(ei));
+        }
+        b->graph_steps = K;
+        b->key = key;
+        b->have_key = true;
+    }
+    int done = 0;
+    if (b->graph)
+        for (; done + K <= nsteps; done += K) HIP_CHECK(hipGraphLaunch(b->graph, b->stream));
+    // the rest, launched directly; projected after the last step
+    for (int it = done; it < nsteps; it++)
+        if (int e = batch_step_all(b, coupling, step, al, reunit_every > 0 && it == nsteps - 1)) return e;
+    b->step += (uint32_t)nsteps;
+    return JW_OK;
+}
+
+int jw_batch_sync(jw_batch* b) {
+    if (!b) return fail(JW_ERR_ARG, "null batch");
+    HIP_CHECK(hipStreamSynchronize(b->stream));
     return JW_OK;
 }
 

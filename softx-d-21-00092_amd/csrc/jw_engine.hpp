@@ -89,6 +89,17 @@ __device__ __forceinline__ int row_of_block(int b) {
     }
 }
 
+// Trajectory blockIdx.z of a batched launch: its planes sit at z times the
+// per-trajectory size of each array, and its noise key is traj + z (the
+// Philox step is read from the device counter when one is given).
+__device__ __forceinline__ NoiseSrc batch_noise(NoiseSrc ns, size_t V) {
+    ns.traj += blockIdx.z;
+    if (ns.inj) ns.inj += (size_t)blockIdx.z * 16 * V;
+    if (ns.stepp) ns.step = *ns.stepp;
+    return ns;
+}
+__device__ __forceinline__ size_t zofs(int planes, size_t V) { return (size_t)blockIdx.z * planes * V; }
+
 // the same pointer, opaque to the optimiser: loads through it are not merged
 // with earlier loads of the same addresses (which would keep every twiddle of
 // one FFT live in registers until the next)
@@ -116,6 +127,8 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_noise_f
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
     __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
+    Xi += zofs(8, V);
+    ns = batch_noise(ns, V);
     const int x = row_of_block<N, BF>(blockIdx.x);
     const size_t row0 = (size_t)x * N;
     T m[SPT][8];
@@ -162,6 +175,9 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(C* __restrict__ Vv
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
     __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
+    Vv += zofs(8, V);
+    U += zofs(9, V);
+    ns = batch_noise(ns, V);
     const int x = row_of_block<N, BF>(blockIdx.x);
     const size_t row0 = (size_t)x * N;
     T vy[SPT][8];      // in the element type: they go to LDS in it
@@ -222,6 +238,9 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
     constexpr int PN = padded<N, 4>::value;
     __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
+    Vv += zofs(8, V);
+    U += zofs(9, V);
+    Xs += zofs(8, V);
     const int x = row_of_block<N, BF>(blockIdx.x);
     const size_t row0 = (size_t)x * N;
 #pragma unroll 1
@@ -288,6 +307,9 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
     const int tid = threadIdx.x;
     const int strip = xcd_strip(blockIdx.x, N / W);
     const int col0 = strip * W;
+    Z += zofs(4, V);
+    X += zofs(8, V);
+    Y += zofs(8, V);
     const size_t pofs = (size_t)blockIdx.y * V;
     const double invV = 1.0 / ((double)N * (double)N);
     // load both tiles: element e -> row e / 2W, slot e % 2W (X for slot < W)
@@ -390,6 +412,9 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __res
     constexpr int PN = padded<N, 4>::value;
     __shared__ C lds[CPB * PN];
     const int tid = threadIdx.x;
+    Z += zofs(4, V);
+    X += zofs(8, V);
+    Y += zofs(8, V);
     const int y0 = blockIdx.x * CPB;
     const double invV = 1.0 / ((double)N * (double)N);
     // global offset within a plane / LDS slot of loaded element e: x-block
@@ -489,6 +514,9 @@ __global__ __launch_bounds__((RowE<N, 4>::NT)) void k_row_update(C* __restrict__
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
     __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
+    U += zofs(9, V);
+    ZA += zofs(4, V);
+    ZB += zofs(4, V);
     const int x = row_of_block<N, BF>(blockIdx.x);
     const size_t row0 = (size_t)x * N;
 #pragma unroll
@@ -548,25 +576,26 @@ static const C* twiddles(const jw_ctx* c) {
 }
 
 template <int N, int BF, typename C>
-static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int sqrt_coupling, int corr) {
+static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int sqrt_coupling, int corr,
+                           int nbatch) {
     const size_t V = c->V;
     const C* tw = twiddles<C>(c);
     if constexpr (BF == 0) {
         constexpr int W = ColE<N, C>::W;
         if (corr)
-            hipLaunchKernelGGL((k_col_kmult<N, 1, C>), dim3(N / W, 4), dim3(ColE<N, C, 1>::NT), 0, c->stream, Z, X,
+            hipLaunchKernelGGL((k_col_kmult<N, 1, C>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 1>::NT), 0, c->stream, Z, X,
                                Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
         else
-            hipLaunchKernelGGL((k_col_kmult<N, 0, C>), dim3(N / W, 4), dim3(ColE<N, C, 0>::NT), 0, c->stream, Z, X,
+            hipLaunchKernelGGL((k_col_kmult<N, 0, C>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 0>::NT), 0, c->stream, Z, X,
                                Y, V,
                                sqrt_coupling, c->q4, c->sb, sa_k, tw);
     } else {
         constexpr int CPB = ColB<N, BF, C>::CPB, NT = ColB<N, BF, C>::NT;
         if (corr)
-            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1, C>), dim3(N / CPB, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
+            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1, C>), dim3(N / CPB, 4, nbatch), dim3(NT), 0, c->stream, Z, X, Y, V,
                                sqrt_coupling, c->q4, c->sb, sa_k, tw);
         else
-            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 0, C>), dim3(N / CPB, 4), dim3(NT), 0, c->stream, Z, X, Y, V,
+            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 0, C>), dim3(N / CPB, 4, nbatch), dim3(NT), 0, c->stream, Z, X, Y, V,
                                sqrt_coupling, c->q4, c->sb, sa_k, tw);
     }
     return hipGetLastError();
@@ -579,31 +608,31 @@ static hipError_t step_fused_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, cons
     hipError_t e;
     {
         KTimer t(c, kcls[0]);
-        hipLaunchKernelGGL((k_row_noise_fwd<N, BF, C>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Xi, V, a.ns,
+        hipLaunchKernelGGL((k_row_noise_fwd<N, BF, C>), dim3(N, 1, a.nbatch), dim3(RowE<N>::NT), 0, c->stream, Xi, V, a.ns,
                            a.sa_pos, tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[1]);
-        if ((e = col_pass<N, BF>(c, ZA, Xi, Xi + 4 * V, a.sa_k, a.sqrt_coupling, a.corr)) != hipSuccess) return e;
+        if ((e = col_pass<N, BF>(c, ZA, Xi, Xi + 4 * V, a.sa_k, a.sqrt_coupling, a.corr, a.nbatch)) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[2]);
         if (a.corr)
-            hipLaunchKernelGGL((k_row_uxiu_corr<N, BF, C>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, Xi, V,
+            hipLaunchKernelGGL((k_row_uxiu_corr<N, BF, C>), dim3(N, 1, a.nbatch), dim3(RowE<N>::NT), 0, c->stream, Vv, U, Xi, V,
                                tw);
         else
-            hipLaunchKernelGGL((k_row_uxiu_fwd<N, BF, C>), dim3(N), dim3(RowE<N>::NT), 0, c->stream, Vv, U, V, a.ns,
+            hipLaunchKernelGGL((k_row_uxiu_fwd<N, BF, C>), dim3(N, 1, a.nbatch), dim3(RowE<N>::NT), 0, c->stream, Vv, U, V, a.ns,
                                a.sa_pos, tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[1]);
-        if ((e = col_pass<N, BF>(c, ZB, Vv, Vv + 4 * V, a.sa_k, a.sqrt_coupling, 0)) != hipSuccess) return e;
+        if ((e = col_pass<N, BF>(c, ZB, Vv, Vv + 4 * V, a.sa_k, a.sqrt_coupling, 0, a.nbatch)) != hipSuccess) return e;
     }
     {
         KTimer t(c, kcls[3]);
-        hipLaunchKernelGGL((k_row_update<N, BF, C>), dim3(N), dim3(RowE<N, 4>::NT), 0, c->stream, U, ZA, ZB, V,
+        hipLaunchKernelGGL((k_row_update<N, BF, C>), dim3(N, 1, a.nbatch), dim3(RowE<N, 4>::NT), 0, c->stream, U, ZA, ZB, V,
                            a.sqrt_step, a.reunit, tw);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }

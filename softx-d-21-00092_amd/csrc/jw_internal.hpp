@@ -26,6 +26,7 @@ typedef float2 cplxf;
 struct NoiseSrc {
     const double* inj;     // device pointer or nullptr -> Philox
     uint32_t seed, traj, step, tag;
+    const uint32_t* stepp; // device step counter (read instead of step when set)
 };
 
 struct Alpha {
@@ -122,10 +123,12 @@ struct StepArgs {
     int corr;                // per-mode correlated noise
     double sqrt_step;        // sqrt(Delta s)
     int reunit;              // project the updated U onto SU(3)
+    int nbatch;              // trajectories per launch (grid z; 1 for a single lattice)
 };
 hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a);
 hipError_t step_fused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a);
 hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32);
+hipError_t step_inc(hipStream_t s, uint32_t* step);
 hipError_t block16(jw_ctx* c, double* out, const double* in);
 
 // observables

@@ -42,4 +42,15 @@ hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32) {
     return hipGetLastError();
 }
 
+// the device Philox step counter of a batch, advanced once per step (inside the
+// captured step graph)
+__global__ void k_step_inc(uint32_t* step) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) step[0] += 1u;
+}
+
+hipError_t step_inc(hipStream_t s, uint32_t* step) {
+    hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, s, step);
+    return hipGetLastError();
+}
+
 }  // namespace jw

@@ -84,6 +84,19 @@ def lib():
         L.jw_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_set_fused.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_calibration_copy.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.jw_batch_destroy.restype = None
+        L.jw_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.jw_batch_size.argtypes = [ctypes.c_void_p]
+        L.jw_batch_set.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.jw_batch_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.jw_batch_philox.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.jw_batch_set_layout.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.jw_batch_layout.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_batch_evolve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.POINTER(Alpha), ctypes.c_int, ctypes.c_int]
+        L.jw_batch_sync.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -261,6 +274,54 @@ class Context:
         out = np.zeros((h8.shape[0], 9), np.complex128)
         _check(lib().jw_su3_exp_gen(self.h, _p(h8), s, _p(out), h8.shape[0]))
         return out.reshape(-1, 3, 3)
+
+
+class Batch:
+    """Independent trajectories in size groups (jw_batch_*): groups = [(N, count), ...]."""
+
+    def __init__(self, groups, fp32=True, device=0):
+        n = len(groups)
+        Ns = (ctypes.c_int * n)(*[g[0] for g in groups])
+        cs = (ctypes.c_int * n)(*[g[1] for g in groups])
+        h = ctypes.c_void_p()
+        _check(lib().jw_batch_create(ctypes.byref(h), device, n, Ns, cs, 1 if fp32 else 0))
+        self.h, self.groups = h, list(groups)
+
+    def close(self):
+        if self.h:
+            lib().jw_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return lib().jw_batch_size(self.h)
+
+    def set(self, t, U):
+        _check(lib().jw_batch_set(self.h, t, U.h))
+
+    def get(self, t, U):
+        _check(lib().jw_batch_get(self.h, t, U.h))
+
+    def philox(self, seed, traj0=0, step=0):
+        _check(lib().jw_batch_philox(self.h, seed, traj0, step))
+
+    def set_layout(self, group, layout):
+        _check(lib().jw_batch_set_layout(self.h, group, layout))
+
+    def layout(self, group):
+        return lib().jw_batch_layout(self.h, group)
+
+    def evolve(self, coupling=FIXED_COUPLING, kernel=SIN_KERNEL, step=1e-4, al=None, nsteps=1, reunit_every=10):
+        _check(lib().jw_batch_evolve(self.h, coupling, kernel, step, ctypes.byref(al) if al else None, nsteps,
+                                     reunit_every))
+
+    def sync(self):
+        _check(lib().jw_batch_sync(self.h))
 
 
 def radial_bins(S, N, nbins):

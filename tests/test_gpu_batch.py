@@ -1,0 +1,143 @@
+"""GPU tests of the batched form (jw_batch_*): trajectories of uneven sizes,
+each against the same trajectory evolved alone with the matching Philox key.
+
+With the same intermediate layout the batched kernels do, per trajectory, the
+same arithmetic as the single-lattice ones, so the results agree to the last
+bit (checked at 1e-13).  The step graph is replayed several times with
+downloads and eager work in between, each replay checked against the eager
+single-lattice result.  With the tuner's layout the agreement is to rounding
+(FP32: 2e-6 after a few steps).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def frob_max(a, b):
+    d = (a - b).reshape(-1, 9)
+    return np.sqrt((abs(d) ** 2).sum(axis=1)).max()
+
+
+def mv_states(jw, groups, Ny=4):
+    """One MV start per trajectory (own seed), as host arrays, and per-N contexts."""
+    ctxs, states = {}, []
+    t = 0
+    for N, count in groups:
+        if N not in ctxs:
+            ctxs[N] = jw.Context(N)
+        c = ctxs[N]
+        for _ in range(count):
+            U = c.field()
+            c.philox(100 + t, 0, 0)
+            c.mv_init(U, Ny, 1.0, 0.25, 1.0)
+            states.append((N, U.download()))
+            t += 1
+    return ctxs, states
+
+
+def single(jw, c, U0, fp32, seed, traj, step0, nsteps, coupling=3, ds=1e-4, al=None, reunit=10):
+    U = c.field(9, U0)
+    c.philox(seed, traj, step0)
+    if fp32:
+        c.evolve_mixed(U, coupling, 1, ds, al, nsteps=nsteps, reunit_every=reunit)
+    else:
+        c.evolve(U, coupling, 1, ds, al, nsteps=nsteps)
+    return U.download()
+
+
+@pytest.mark.parametrize("fp32", [1, 0])
+def test_batch_matches_single_trajectories(jw, fp32):
+    groups = [(64, 3), (128, 2)]
+    ctxs, states = mv_states(jw, groups)
+    b = jw.Batch(groups, fp32=fp32)
+    assert len(b) == 5
+    for g in range(len(groups)):
+        b.set_layout(g, 0)
+    for t, (N, U0) in enumerate(states):
+        b.set(t, ctxs[N].field(9, U0))
+    b.philox(7, 5, 0)
+    # 25 steps: the 10-step graph twice, then 5 steps launched directly
+    b.evolve(3, 1, 1e-4, None, nsteps=25, reunit_every=10)
+    b.sync()
+    for t, (N, U0) in enumerate(states):
+        c = ctxs[N]
+        got = c.field()
+        b.get(t, got)
+        ref = single(jw, c, U0, fp32, 7, 5 + t, 0, 25)
+        assert frob_max(got.download(), ref) <= 1e-13, t
+        assert frob_max(ref, U0) > 1e-3
+    b.close()
+    for c in ctxs.values():
+        c.close()
+
+
+def test_batch_graph_replays(jw):
+    """Three evolutions of 10 steps (one graph replay each) with reads and eager
+    single-lattice work in between: every replay matches."""
+    import math
+    groups = [(64, 2), (256, 1)]
+    ctxs, states = mv_states(jw, groups)
+    al = jw.alpha(lambda_L=0.02 * 64, mu0_over_lambda=math.exp(2 * math.pi / (9 * 0.7)))
+    b = jw.Batch(groups, fp32=True)
+    for g in range(len(groups)):
+        b.set_layout(g, 0)
+    for t, (N, U0) in enumerate(states):
+        b.set(t, ctxs[N].field(9, U0))
+    b.philox(3, 0, 0)
+    cur = [U0 for _, U0 in states]
+    for rep in range(3):
+        b.evolve(4, 1, 5e-5, al, nsteps=10, reunit_every=10)
+        for t, (N, _) in enumerate(states):
+            c = ctxs[N]
+            # the single-lattice reference for this replay, from the previous state
+            ref = single(jw, c, cur[t], 1, 3, t, 10 * rep, 10, coupling=4, ds=5e-5, al=al)
+            got = c.field()
+            b.get(t, got)
+            g = got.download()
+            assert frob_max(g, ref) <= 1e-13, (rep, t)
+            cur[t] = ref
+    b.close()
+    for c in ctxs.values():
+        c.close()
+
+
+def test_batch_tuned_layout(jw):
+    groups = [(256, 2), (512, 1), (64, 4)]
+    ctxs, states = mv_states(jw, groups, Ny=2)
+    b = jw.Batch(groups, fp32=True)
+    for t, (N, U0) in enumerate(states):
+        b.set(t, ctxs[N].field(9, U0))
+    b.philox(11, 0, 0)
+    b.evolve(3, 1, 1e-4, None, nsteps=12, reunit_every=10)
+    b.sync()
+    for g in range(len(groups)):
+        assert b.layout(g) in (0, 4, 8)
+    for t, (N, U0) in enumerate(states):
+        c = ctxs[N]
+        got = c.field()
+        b.get(t, got)
+        ref = single(jw, c, U0, 1, 11, t, 0, 12)
+        assert frob_max(got.download(), ref) <= 2e-6, t
+    b.close()
+    for c in ctxs.values():
+        c.close()
+
+
+def test_batch_rejects(jw):
+    with pytest.raises(jw.JwError):
+        jw.Batch([(32, 2)])
+    with pytest.raises(jw.JwError):
+        jw.Batch([(64, 0)])
+    b = jw.Batch([(64, 1)])
+    with pytest.raises(jw.JwError):
+        b.evolve(3, 0, 1e-4, None, 1)          # LINEAR kernel
+    with pytest.raises(jw.JwError):
+        b.evolve(2, 1, 1e-4, None, 1)          # HATTA
+    c = jw.Context(128)
+    with pytest.raises(jw.JwError):
+        b.set(0, c.field())                    # wrong N
+    with pytest.raises(jw.JwError):
+        b.set(1, c.field())                    # no such trajectory
+    b.close()
+    c.close()
