@@ -12,10 +12,15 @@ Synthetic data: all noise comes from the Philox stream.
 Forms, each timed over the same K steps from the same start:
   * fp64  -- jw_evolve: FP64 throughout (the oracle-parity path);
   * mixed -- jw_evolve_mixed: FP32 fields and FFTs, FP64 exponentials, U
-             projected onto SU(3) every 10 steps.
-`value` is the mixed form's steps/s; the FP64 form's steps/s, the ratio and
-S(r) after --sr-steps steps of both forms from the same MV start and Philox
-stream (max |S_mixed(r) - S_fp64(r)| over all lattice separations r) sit beside it.
+             projected onto SU(3) every 10 steps;
+  * mixed+batched -- jw_batch_evolve with --batch N = 2048 trajectories in one
+             batch (FP32, captured step graph, tuned layout).
+`value` is the mixed+batched form's trajectory-steps/s (whole job); the FP64
+form's steps/s, the ratio and S(r) after --sr-steps steps of the FP64 and
+mixed forms from the same MV start and Philox stream (max |S_mixed(r) -
+S_fp64(r)| over all lattice separations r) sit beside it.  `ensemble` times
+the uneven batched ensemble 48 x 256^2 + 32 x 512^2 + 16 x 1024^2 (fixed
+coupling, Delta s = 1e-4) in both precisions, in site-updates/s.
 
 One step = noise generation, 2-D FFTs, kernel multiplies, U xi U^dagger and
 the SU(3) update; measurement is outside the timed region, as the reference's
@@ -76,6 +81,8 @@ def parse():
     p.add_argument("--fused-mode", type=int, default=1,
                    help="jw_set_fused mode: 1 default, 2/3/4 row-major/4-row/8-row blocked intermediates")
     p.add_argument("--forms", default="fp64,mixed", help="forms to time (profiling runs may pick one)")
+    p.add_argument("--batch", type=int, default=2, help="N = 2048 trajectories of the batched form (0: skip)")
+    p.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble")
     return p.parse_args()
 
 
@@ -171,6 +178,54 @@ def main():
                                        "reals_per_site": REALS_PER_SITE[kind], "bytes_per_real": bpr}}
         F.close()
 
+    def time_batch(groups, fp32, coupling_b, step_b, al_b, nsteps, warm, seeds0):
+        """Aggregate rate of a batch of MV starts (own seed each): (ensemble steps/s, layouts)."""
+        b = jw.Batch(groups, fp32=fp32)
+        ctxs = {}
+        t = 0
+        for N_, count in groups:
+            c_ = ctxs.setdefault(N_, jw.Context(N_))
+            F_ = c_.field()
+            for _ in range(count):
+                c_.philox(seeds0 + t, 0, 0)
+                c_.mv_init(F_, 50, 1.0, g2mu, 1e-4 / g2mu)
+                b.set(t, F_)
+                t += 1
+            F_.close()
+        b.philox(4, 0, 0)
+        b.evolve(coupling_b, jw.SIN_KERNEL, step_b, al_b, nsteps=warm, reunit_every=a.reunit_every)
+        b.sync()
+        t0 = time.perf_counter()
+        b.evolve(coupling_b, jw.SIN_KERNEL, step_b, al_b, nsteps=nsteps, reunit_every=a.reunit_every)
+        b.sync()
+        dt = time.perf_counter() - t0
+        lays = [b.layout(i) for i in range(len(groups))]
+        b.close()
+        for c_ in ctxs.values():
+            c_.close()
+        return nsteps / dt, lays
+
+    batched = None
+    if a.batch > 0:
+        sps_b, lays = time_batch([(N, a.batch)], True, coupling, step, al, a.steps, a.warmup, 1000)
+        batched = {"trajectories": a.batch, "N": N, "ensemble_steps_per_s": round(sps_b, 3),
+                   "traj_steps_per_s": round(sps_b * a.batch, 3), "layout": lays[0]}
+        print(f"bench: mixed+batched: {a.batch} x N={N}: {sps_b * a.batch:.1f} trajectory-steps/s "
+              f"(layout {lays[0]})", file=sys.stderr, flush=True)
+    ensemble = None
+    if not a.no_ensemble:
+        groups = [(256, 48), (512, 32), (1024, 16)]
+        sites = sum(n * n * c for n, c in groups)
+        ensemble = {"groups": [{"N": n, "count": c} for n, c in groups], "sites": sites,
+                    "coupling": "fixed", "step": 1e-4}
+        for fp32, name in ((True, "mixed"), (False, "fp64")):
+            sps_e, lays = time_batch(groups, fp32, jw.FIXED_COUPLING, 1e-4, None, 20, 10, 2000)
+            ensemble[name] = {"steps_per_s": round(sps_e, 3), "site_updates_per_s": round(sps_e * sites, 1),
+                              "layouts": lays}
+            print(f"bench: ensemble {name}: {sps_e:.1f} steps/s = {sps_e * sites:.3e} site-updates/s "
+                  f"(layouts {lays})", file=sys.stderr, flush=True)
+        ensemble["mixed_over_fp64"] = round(ensemble["mixed"]["steps_per_s"] / ensemble["fp64"]["steps_per_s"], 3)
+
     # measured traffic of the dominant kernels, when a profile summary is present
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -225,20 +280,28 @@ def main():
 
     head = "mixed" if "mixed" in res else forms[0]
     h = res[head]
-    out = {"metric": METRIC, "value": h["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
-           "warmup": a.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+    value = batched["traj_steps_per_s"] if batched else h["steps_per_s"]
+    form_txt = (f"mixed+batched: {a.batch} trajectories of N={N} in one batch, FP32 fields and FFTs, FP64 "
+                f"exponentials, SU(3) projection every {a.reunit_every} steps" if batched else
+                ("mixed: FP32 fields and FFTs, FP64 exponentials, SU(3) projection every "
+                 f"{a.reunit_every} steps; one trajectory") if head == "mixed" else "plain FP64")
+    out = {"metric": METRIC, "value": value, "unit": "trajectory-steps/s" if batched else "steps/s",
+           "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": round(1e3 / value, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32" if head == "mixed" else "f64",
            "data": "synthetic (Philox noise, MV initial condition)",
            "config": {"workload": f"N={N} Langevin step, {a.coupling} coupling"
                                   + (f" (alpha_max={ALPHA_MAX}, Lambda a={LAMBDA_A})" if a.coupling == "correlated" else "")
                                   + f", SIN kernel, ds={step}, MV start Ny=50 g2mu a=0.25 m a=1e-4",
                       "N": N, "sites": V,
-                      "form": ("mixed: FP32 fields and FFTs, FP64 exponentials, SU(3) projection every "
-                               f"{a.reunit_every} steps; one trajectory") if head == "mixed" else "plain FP64",
-                      "parallelism": "single GPU"},
+                      "form": form_txt, "parallelism": "single GPU"},
            "fp64_steps_per_s": res["fp64"]["steps_per_s"] if "fp64" in res else None,
+           "mixed_steps_per_s": res["mixed"]["steps_per_s"] if "mixed" in res else None,
+           "mixed_batched": batched,
+           "value_over_fp64": round(value / res["fp64"]["steps_per_s"], 3) if "fp64" in res else None,
            "mixed_over_fp64": round(res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"], 3)
            if "mixed" in res and "fp64" in res else None,
+           "ensemble": ensemble,
            "sr_err_vs_fp64": sr,
            "roofline": h["roofline"], "step_roofline": h["step_roofline"], "kernels": h["kernels"],
            "roofline_fp64": res["fp64"]["roofline"] if "fp64" in res else None,
