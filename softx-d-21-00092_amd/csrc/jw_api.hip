@@ -828,8 +828,8 @@ static const char* kNames[] = {"fft_rows", "fft_cols", "gen_noise", "kernel_mult
 
 int jw_set_fused(jw_ctx* c, int mode) {
     if (!c) return fail(JW_ERR_ARG, "null context");
-    static const int layouts[5] = {0, JW_DEFAULT_LAYOUT, 0, 4, 8};
-    if (mode < 0 || mode > 4) return fail(JW_ERR_ARG, "fused mode %d: expected 0..4", mode);
+    static const int layouts[6] = {0, JW_DEFAULT_LAYOUT, 0, 4, 8, 1};
+    if (mode < 0 || mode > 5) return fail(JW_ERR_ARG, "fused mode %d: expected 0..5", mode);
     c->unfused = mode ? 0 : 1;
     c->layout = layouts[mode];
     return JW_OK;

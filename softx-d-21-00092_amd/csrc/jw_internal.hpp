@@ -69,8 +69,8 @@ struct jw_ctx {
     size_t ws_planes;
     // 1: evolve through the unfused kernel sequence even where the fused passes exist
     int unfused;
-    // fused step's intermediate-plane layout: 0 row-major, 4 or 8 rows per block,
-    // -1 default per step kind
+    // fused step's intermediate-plane layout: 0 row-major (LDS-strip column
+    // pass), 1 row-major (register-FFT column pass), 4 or 8 rows per block, -1 default
     int layout;
     // per-kernel timing (HIP events on ctx->stream)
     int timing;
