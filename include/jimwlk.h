@@ -148,7 +148,8 @@ int jw_evolve(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, c
  * converted to FP32 on entry and back on return, and projected onto SU(3)
  * every reunit_every steps and after the last step (reunit_every = 0: never).
  * The noise is the same Philox stream (normals computed in FP32 from the same
- * words) or the pending injection.  SIN kernel, 64 <= N <= 2048. */
+ * words) or the pending injection.  SIN kernel; the fused passes for
+ * 64 <= N <= 2048, the unfused ones otherwise. */
 int jw_evolve_mixed(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
                     int nsteps, int reunit_every);
 

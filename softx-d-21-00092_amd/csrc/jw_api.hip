@@ -121,16 +121,7 @@ int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, jw::c
             HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, sa));
             continue;
         }
-        HIP_CHECK(jw::gen_noise_packed(c, Xi, ns, sa_pos));
-        HIP_CHECK(jw::fft2d(c, Xi, 8, 1, s));
-        HIP_CHECK(jw::kernel_mult_packed(c, ZA, Xi, Xi + 4 * V, kc, sa_k, corr));
-        HIP_CHECK(jw::fft2d(c, ZA, 4, 0, s));
-        if (corr) HIP_CHECK(jw::fft2d(c, Xi, 8, 0, s));
-        HIP_CHECK(jw::uxiu_packed(c, Vv, U->d, corr ? Xi : nullptr, ns, sa_pos));
-        HIP_CHECK(jw::fft2d(c, Vv, 8, 1, s));
-        HIP_CHECK(jw::kernel_mult_packed(c, ZB, Vv, Vv + 4 * V, kc, sa_k, 0));
-        HIP_CHECK(jw::fft2d(c, ZB, 4, 0, s));
-        HIP_CHECK(jw::update_packed(c, U->d, ZA, ZB, ss));
+        HIP_CHECK(jw::step_unfused(c, U->d, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc, corr, ss));
     }
     return JW_OK;
 }
@@ -673,7 +664,6 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
         return fail(JW_ERR_UNSUPPORTED, "HATTA coupling: the reference leaves the coupling uninitialised here");
     if (coupling < 0 || coupling > 4 || (kernel != 0 && kernel != 1)) return fail(JW_ERR_ARG, "bad coupling/kernel");
     if (kernel != JW_SIN_KERNEL) return fail(JW_ERR_UNSUPPORTED, "the mixed-precision form runs the SIN kernel");
-    if (!jw::fused_supported(c->N)) return fail(JW_ERR_UNSUPPORTED, "the mixed-precision form needs 64 <= N <= 2048");
     if (!(step >= 0)) return fail(JW_ERR_ARG, "step must be >= 0");
     if (nsteps < 0 || reunit_every < 0) return fail(JW_ERR_ARG, "nsteps or reunit_every < 0");
     if (c->inj_per_site && nsteps > 1) return fail(JW_ERR_STATE, "an injection covers one step; nsteps = %d", nsteps);
@@ -697,10 +687,11 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
         sa_k = c->sa;
     }
     const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
+    const bool fused = jw::fused_supported(c->N) && !c->unfused;
     const int saved_layout = c->layout;
     if (c->layout < 0) c->layout = default_layout(c->N, corr, 1);
     int rc = JW_OK;
-    if (sa_k && c->layout != 0) {
+    if (fused && sa_k && c->layout != 0) {
         rc = ensure_sa16(c);
         sa_k = c->sa16;
     }
@@ -715,7 +706,10 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
         // projection every reunit_every steps and after the last one
         const int reunit = reunit_every > 0 && ((it + 1) % reunit_every == 0 || it == nsteps - 1);
         const jw::StepArgs sa{ns, sa_pos, sa_k, coupling == JW_SQRT_COUPLING_CONSTANT, corr, sqrt(step), reunit, 1};
-        const hipError_t he = jw::step_fused_f32(c, U32, Xi, Vv, ZA, ZB, sa);
+        const hipError_t he = fused ? jw::step_fused_f32(c, U32, Xi, Vv, ZA, ZB, sa)
+                                    : jw::step_unfused_f32(c, U32, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k,
+                                                           coupling == JW_SQRT_COUPLING_CONSTANT ? 0 : 1, corr,
+                                                           sqrt(step), reunit);
         if (he != hipSuccess) rc = fail(JW_ERR_HIP, "%s", hipGetErrorString(he));
     }
     if (rc == JW_OK && jw::u_convert(c, U32, U->d, 0) != hipSuccess) rc = fail(JW_ERR_HIP, "U conversion failed");

@@ -102,13 +102,11 @@ hipError_t average_and_symmetrize(jw_ctx* c, const cplx* in, cplx* out);
 hipError_t reduce(jw_ctx* c, const cplx* corr, cplx* sum, cplx* err);
 hipError_t scale_planes(jw_ctx* c, cplx* p, size_t n, double s);
 
-// generator-basis (packed) engine kernels
-hipError_t gen_noise_packed(jw_ctx* c, cplx* Xi, const NoiseSrc& ns, const double* sa_pos);
-hipError_t kernel_mult_packed(jw_ctx* c, cplx* Z, const cplx* X, const cplx* Y, int coupling,
-                              const double* sa_k, int scale_inputs);
-hipError_t uxiu_packed(jw_ctx* c, cplx* Vv, const cplx* U, const cplx* Xi_pos, const NoiseSrc& ns,
-                       const double* sa_pos);
-hipError_t update_packed(jw_ctx* c, cplx* U, const cplx* ZA, const cplx* ZB, double sqrt_step);
+// generator-basis engine: the unfused step (kc: 0 for SQRT coupling, 1 otherwise)
+hipError_t step_unfused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
+                        const double* sa_pos, const double* sa_k, int kc, int corr, double ss);
+hipError_t step_unfused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const NoiseSrc& ns,
+                            const double* sa_pos, const double* sa_k, int kc, int corr, double ss, int reunit);
 hipError_t mv_noise_packed(jw_ctx* c, cplx* R, const NoiseSrc& ns, double sigma);
 hipError_t poisson_packed(jw_ctx* c, cplx* R, double mass, double g);
 hipError_t mul_exp_packed(jw_ctx* c, cplx* U, const cplx* R);

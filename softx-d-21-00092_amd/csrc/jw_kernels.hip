@@ -11,118 +11,25 @@
 //  * the generator-basis engine: every su(3) field (xi_x, xi_y, A, V_x, V_y, B)
 //    is carried as 8 real generator planes packed two per complex plane, which
 //    turns the reference's 72 complex plane FFTs per step into 24.
-#include "jw_common.hpp"
+#include "jw_packed.hpp"
 
 #include <math.h>
 
 namespace jw {
 
 // ===========================================================================
-// FFT passes
+// FFT passes (jw_packed.hpp), FP64 instances
 // ===========================================================================
-// Rows per workgroup (at most N: a workgroup never reaches past its plane) and
-// columns per workgroup (at most N), each a divisor of N.
-template <int N>
-struct RowCfg {
-    static constexpr int RPW0 = (N >= 1024) ? 1 : (1024 / N);
-    static constexpr int RPW = RPW0 < N ? RPW0 : N;
-    static constexpr int ELEMS = RPW * N;
-    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 1024) ? 1024 : ELEMS / 8);
-    static_assert(RPW >= 1 && RPW <= N && N % RPW == 0, "row tiling must divide the plane");
-};
-template <int N>
-struct ColCfg {
-    static constexpr int W0 = (N >= 4096) ? 1 : ((4096 / N > 16) ? 16 : 4096 / N);
-    static constexpr int W = W0 < N ? W0 : N;
-    static constexpr int ELEMS = W * N;
-    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 1024) ? 1024 : ELEMS / 8);
-    static_assert(W >= 1 && W <= N && N % W == 0, "column tiling must divide the plane");
-};
-
-template <int N, int DIR>
-__global__ __launch_bounds__(RowCfg<N>::NT) void k_fft_rows(cplx* __restrict__ data, size_t plane_stride,
-                                                             const cplx* __restrict__ tw, double scale) {
-    using C = RowCfg<N>;
-    constexpr int PN = padded<N, 4>::value;            // padded row length in LDS
-    __shared__ cplx lds[C::RPW * PN];
-    const int tid = threadIdx.x;
-    cplx* base = data + blockIdx.y * plane_stride + (size_t)blockIdx.x * C::ELEMS;
-    for (int i = tid; i < C::ELEMS; i += C::NT) lds[lpos<1, PN, 4>(i % N, i / N)] = base[i];
-    __syncthreads();
-    lds_fft<N, C::RPW, 1, PN, C::NT, DIR, 4>(lds, tw, tid);
-    for (int i = tid; i < C::ELEMS; i += C::NT) base[i] = cscale(lds[lpos<1, PN, 4>(i % N, i / N)], scale);
-}
-
-template <int N, int DIR>
-__global__ __launch_bounds__(ColCfg<N>::NT) void k_fft_cols(cplx* __restrict__ data, size_t plane_stride,
-                                                             const cplx* __restrict__ tw, double scale) {
-    using C = ColCfg<N>;
-    __shared__ cplx lds[padded<N, 3>::value * C::W];
-    const int tid = threadIdx.x;
-    cplx* base = data + blockIdx.y * plane_stride + (size_t)blockIdx.x * C::W;
-    for (int i = tid; i < C::ELEMS; i += C::NT)
-        lds[lpos<C::W, 1, 3>(i / C::W, i % C::W)] = base[(size_t)(i / C::W) * N + (i % C::W)];
-    __syncthreads();
-    lds_fft<N, C::W, C::W, 1, C::NT, DIR, 3>(lds, tw, tid);
-    for (int i = tid; i < C::ELEMS; i += C::NT)
-        base[(size_t)(i / C::W) * N + (i % C::W)] = cscale(lds[lpos<C::W, 1, 3>(i / C::W, i % C::W)], scale);
-}
-
-template <int N>
-static hipError_t launch_rows(jw_ctx* c, cplx* p, int np, int dir, double scale) {
-    using C = RowCfg<N>;
-    dim3 grid(N / C::RPW, np);
-    KTimer t(c, K_FFT_ROWS);
-    if (dir) hipLaunchKernelGGL((k_fft_rows<N, -1>), grid, dim3(C::NT), 0, c->stream, p, c->V, c->tw, scale);
-    else hipLaunchKernelGGL((k_fft_rows<N, +1>), grid, dim3(C::NT), 0, c->stream, p, c->V, c->tw, scale);
-    return hipGetLastError();
-}
-
-template <int N>
-static hipError_t launch_cols(jw_ctx* c, cplx* p, int np, int dir, double scale) {
-    using C = ColCfg<N>;
-    dim3 grid(N / C::W, np);
-    KTimer t(c, K_FFT_COLS);
-    if (dir) hipLaunchKernelGGL((k_fft_cols<N, -1>), grid, dim3(C::NT), 0, c->stream, p, c->V, c->tw, scale);
-    else hipLaunchKernelGGL((k_fft_cols<N, +1>), grid, dim3(C::NT), 0, c->stream, p, c->V, c->tw, scale);
-    return hipGetLastError();
-}
-
-#define JW_SIZE_SWITCH(N, CALL)                         \
-    switch (N) {                                        \
-        case 4: return CALL(4);                         \
-        case 8: return CALL(8);                         \
-        case 16: return CALL(16);                       \
-        case 32: return CALL(32);                       \
-        case 64: return CALL(64);                       \
-        case 128: return CALL(128);                     \
-        case 256: return CALL(256);                     \
-        case 512: return CALL(512);                     \
-        case 1024: return CALL(1024);                   \
-        case 2048: return CALL(2048);                   \
-        case 4096: return CALL(4096);                   \
-        case 8192: return CALL(8192);                   \
-        default: return hipErrorInvalidValue;           \
-    }
-
 hipError_t fft_rows(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale) {
-#define CALL_ROWS(n) launch_rows<n>(c, planes, nplanes, dir, scale)
-    JW_SIZE_SWITCH(c->N, CALL_ROWS)
-#undef CALL_ROWS
+    return fft_rows_t(c, planes, nplanes, dir, scale);
 }
 
 hipError_t fft_cols(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale) {
-#define CALL_COLS(n) launch_cols<n>(c, planes, nplanes, dir, scale)
-    JW_SIZE_SWITCH(c->N, CALL_COLS)
-#undef CALL_COLS
+    return fft_cols_t(c, planes, nplanes, dir, scale);
 }
 
-// unitary 2-D transform of nplanes planes: dir 1 -> exp(-i k.x), dir 0 -> exp(+i k.x),
-// scaled by `scale` once (1/N for the reference's 1/sqrt(V))
 hipError_t fft2d(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale) {
-    hipError_t e = fft_rows(c, planes, nplanes, dir, 1.0);
-    if (e != hipSuccess) return e;
-    return fft_cols(c, planes, nplanes, dir, scale);
+    return fft2d_t(c, planes, nplanes, dir, scale);
 }
 
 // ===========================================================================
@@ -451,124 +358,11 @@ hipError_t scale_planes(jw_ctx* c, cplx* p, size_t n, double s) {
 }
 
 // ===========================================================================
-// generator-basis engine (packed planes)
-//   Xi planes 0..3: (n0,n1),(n2,n3),(n4,n5),(n6,n7) of xi_x; planes 4..7: same for xi_y
-//   Vv planes: the same layout for V_x, V_y;  ZA, ZB: 4 planes (A, B)
+// generator-basis engine (jw_packed.hpp), the unfused FP64 step
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_gen_noise_packed(cplx* __restrict__ Xi, size_t V, NoiseSrc ns, const double* __restrict__ sa_pos) {
-    GRID_STRIDE(i, V) {
-        double z[16];
-        normals16(ns, i, z);
-        const double s = sa_pos ? sa_pos[i] : 1.0;
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            Xi[p * V + i] = mk(z[4 * p] * s, z[4 * p + 2] * s);           // n_{2p}, n_{2p+1}
-            Xi[(4 + p) * V + i] = mk(z[4 * p + 1] * s, z[4 * p + 3] * s); // m_{2p}, m_{2p+1}
-        }
-    }
-}
-
-hipError_t gen_noise_packed(jw_ctx* c, cplx* Xi, const NoiseSrc& ns, const double* sa_pos) {
-    KTimer t(c, K_GEN_NOISE);
-    hipLaunchKernelGGL(k_gen_noise_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, Xi, c->V, ns, sa_pos);
-    return hipGetLastError();
-}
-
-// Z[p] = i (kx(k) X[p] + ky(k) Y[p]),  kx = -2 pi c pbarX / phat2, zero at k = 0;
-// scale_inputs: X, Y *= sa(k) first (and written back) for per-mode correlated noise.
-__global__ __launch_bounds__(256) void k_kernel_mult_packed(cplx* __restrict__ Z, cplx* __restrict__ X, cplx* __restrict__ Y, int N,
-                                     int coupling, const double* __restrict__ q4, const double* __restrict__ sb,
-                                     const double* __restrict__ sa, int scale_inputs) {
-    const size_t V = (size_t)N * N;
-    GRID_STRIDE(i, V) {
-        const int ix = (int)(i / N), iy = (int)(i % N);
-        const double ph = q4[ix] + q4[iy];
-        const double sc = (coupling == 0) ? sa[i] : 1.0;
-        const double kxv = (i == 0) ? 0.0 : -2.0 * M_PI * sc * sb[ix] / ph;
-        const double kyv = (i == 0) ? 0.0 : -2.0 * M_PI * sc * sb[iy] / ph;
-        const double si = scale_inputs ? sa[i] : 1.0;
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            cplx x = X[p * V + i], y = Y[p * V + i];
-            if (scale_inputs) {
-                x = cscale(x, si);
-                y = cscale(y, si);
-                X[p * V + i] = x;
-                Y[p * V + i] = y;
-            }
-            const cplx s = mk(kxv * x.x + kyv * y.x, kxv * x.y + kyv * y.y);
-            Z[p * V + i] = cmuli(s);
-        }
-    }
-}
-
-hipError_t kernel_mult_packed(jw_ctx* c, cplx* Z, const cplx* X, const cplx* Y, int coupling, const double* sa_k,
-                              int scale_inputs) {
-    KTimer t(c, K_KMULT);
-    hipLaunchKernelGGL(k_kernel_mult_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, Z,
-                       const_cast<cplx*>(X), const_cast<cplx*>(Y), c->N, coupling, c->q4, c->sb, sa_k, scale_inputs);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(256) void k_uxiu_packed(cplx* __restrict__ Vv, const cplx* __restrict__ U, const cplx* __restrict__ Xi_pos,
-                              size_t V, NoiseSrc ns, const double* __restrict__ sa_pos) {
-    GRID_STRIDE(i, V) {
-        double hx[8], hy[8];
-        if (Xi_pos) {
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                const cplx a = Xi_pos[p * V + i], b = Xi_pos[(4 + p) * V + i];
-                hx[2 * p] = a.x; hx[2 * p + 1] = a.y;
-                hy[2 * p] = b.x; hy[2 * p + 1] = b.y;
-            }
-        } else {
-            double z[16];
-            normals16(ns, i, z);
-            const double s = sa_pos ? sa_pos[i] : 1.0;
-#pragma unroll
-            for (int k = 0; k < 8; k++) { hx[k] = z[2 * k] * s; hy[k] = z[2 * k + 1] * s; }
-        }
-        const m3 u = load9(U, V, i);
-        double vx[8], vy[8];
-        uxiu_gen(u, hx, vx);
-        uxiu_gen(u, hy, vy);
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            Vv[p * V + i] = mk(vx[2 * p], vx[2 * p + 1]);
-            Vv[(4 + p) * V + i] = mk(vy[2 * p], vy[2 * p + 1]);
-        }
-    }
-}
-
-hipError_t uxiu_packed(jw_ctx* c, cplx* Vv, const cplx* U, const cplx* Xi_pos, const NoiseSrc& ns,
-                       const double* sa_pos) {
-    KTimer t(c, K_UXIU);
-    hipLaunchKernelGGL(k_uxiu_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, Vv, U, Xi_pos, c->V, ns,
-                       sa_pos);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(256) void k_update_packed(cplx* __restrict__ U, const cplx* __restrict__ ZA, const cplx* __restrict__ ZB,
-                                size_t V, double ss) {
-    GRID_STRIDE(i, V) {
-        double ha[8], hb[8];
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            const cplx a = ZA[p * V + i], b = ZB[p * V + i];
-            ha[2 * p] = a.x; ha[2 * p + 1] = a.y;
-            hb[2 * p] = b.x; hb[2 * p + 1] = b.y;
-        }
-        const m3 eb = exp_i_gen(hb, ss);
-        const m3 ea = exp_i_gen(ha, -ss);
-        store9(U, V, i, mmul(mmul(eb, load9(U, V, i)), ea));
-    }
-}
-
-hipError_t update_packed(jw_ctx* c, cplx* U, const cplx* ZA, const cplx* ZB, double sqrt_step) {
-    KTimer t(c, K_UPDATE);
-    hipLaunchKernelGGL(k_update_packed, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, U, ZA, ZB, c->V,
-                       sqrt_step);
-    return hipGetLastError();
+hipError_t step_unfused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,
+                        const double* sa_pos, const double* sa_k, int kc, int corr, double ss) {
+    return step_unfused_t(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc, corr, ss, 0);
 }
 
 // MV sheets in the generator basis

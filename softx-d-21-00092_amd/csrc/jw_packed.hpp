@@ -1,0 +1,291 @@
+// jw_packed.hpp -- the unfused passes of the generator-basis step, templated on
+// the storage type C (double2: FP64 form, float2: mixed form): the 2-D FFT
+// row and column passes and the per-site kernels (noise, kernel multiply,
+// U xi U^dagger, update).  Site algebra is FP64 in both forms.  Used where the
+// fused passes are not (N outside 64..2048, the LINEAR kernel's 9-component
+// transforms, jw_set_fused(ctx, 0)).
+#pragma once
+
+#include "jw_common.hpp"
+
+namespace jw {
+
+// ===========================================================================
+// FFT passes
+// ===========================================================================
+// Rows per workgroup (at most N: a workgroup never reaches past its plane) and
+// columns per workgroup (at most N), each a divisor of N.  Column strips are
+// as wide as about 150 KB of LDS allows (at most 16 columns).
+template <int N, typename C = double2>
+struct RowCfg {
+    static constexpr int RPW0 = (N >= 1024) ? 1 : (1024 / N);
+    static constexpr int RPW = RPW0 < N ? RPW0 : N;
+    static constexpr int ELEMS = RPW * N;
+    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 1024) ? 1024 : ELEMS / 8);
+    static_assert(RPW >= 1 && RPW <= N && N % RPW == 0, "row tiling must divide the plane");
+};
+template <int N, typename C = double2>
+struct ColCfg {
+    static constexpr int BUDGET = 150 * 1024;
+    static constexpr int W1 = BUDGET / (padded<N, 3>::value * (int)sizeof(C));
+    static constexpr int W0 = W1 >= 16 ? 16 : (W1 >= 8 ? 8 : (W1 >= 4 ? 4 : (W1 >= 2 ? 2 : 1)));
+    static constexpr int W = W0 < N ? W0 : N;
+    static constexpr int ELEMS = W * N;
+    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 1024) ? 1024 : ELEMS / 8);
+    static_assert(W >= 1 && W <= N && N % W == 0, "column tiling must divide the plane");
+};
+
+template <int N, int DIR, typename C>
+__global__ __launch_bounds__((RowCfg<N, C>::NT)) void k_fft_rows(C* __restrict__ data, size_t plane_stride,
+                                                              const C* __restrict__ tw, double scale) {
+    typedef typename scalar_of<C>::type T;
+    using G = RowCfg<N, C>;
+    constexpr int PN = padded<N, 4>::value;            // padded row length in LDS
+    __shared__ C lds[G::RPW * PN];
+    const int tid = threadIdx.x;
+    C* base = data + blockIdx.y * plane_stride + (size_t)blockIdx.x * G::ELEMS;
+    for (int i = tid; i < G::ELEMS; i += G::NT) lds[lpos<1, PN, 4>(i % N, i / N)] = base[i];
+    __syncthreads();
+    lds_fft<N, G::RPW, 1, PN, G::NT, DIR, 4>(lds, tw, tid);
+    for (int i = tid; i < G::ELEMS; i += G::NT) base[i] = cscale(lds[lpos<1, PN, 4>(i % N, i / N)], (T)scale);
+}
+
+template <int N, int DIR, typename C>
+__global__ __launch_bounds__((ColCfg<N, C>::NT)) void k_fft_cols(C* __restrict__ data, size_t plane_stride,
+                                                              const C* __restrict__ tw, double scale) {
+    typedef typename scalar_of<C>::type T;
+    using G = ColCfg<N, C>;
+    __shared__ C lds[padded<N, 3>::value * G::W];
+    const int tid = threadIdx.x;
+    C* base = data + blockIdx.y * plane_stride + (size_t)blockIdx.x * G::W;
+    for (int i = tid; i < G::ELEMS; i += G::NT)
+        lds[lpos<G::W, 1, 3>(i / G::W, i % G::W)] = base[(size_t)(i / G::W) * N + (i % G::W)];
+    __syncthreads();
+    lds_fft<N, G::W, G::W, 1, G::NT, DIR, 3>(lds, tw, tid);
+    for (int i = tid; i < G::ELEMS; i += G::NT)
+        base[(size_t)(i / G::W) * N + (i % G::W)] = cscale(lds[lpos<G::W, 1, 3>(i / G::W, i % G::W)], (T)scale);
+}
+
+template <int N, typename C>
+static hipError_t launch_rows(jw_ctx* c, C* p, int np, int dir, double scale, const C* tw) {
+    using G = RowCfg<N, C>;
+    dim3 grid(N / G::RPW, np);
+    KTimer t(c, K_FFT_ROWS);
+    if (dir) hipLaunchKernelGGL((k_fft_rows<N, -1, C>), grid, dim3(G::NT), 0, c->stream, p, c->V, tw, scale);
+    else hipLaunchKernelGGL((k_fft_rows<N, +1, C>), grid, dim3(G::NT), 0, c->stream, p, c->V, tw, scale);
+    return hipGetLastError();
+}
+
+template <int N, typename C>
+static hipError_t launch_cols(jw_ctx* c, C* p, int np, int dir, double scale, const C* tw) {
+    using G = ColCfg<N, C>;
+    dim3 grid(N / G::W, np);
+    KTimer t(c, K_FFT_COLS);
+    if (dir) hipLaunchKernelGGL((k_fft_cols<N, -1, C>), grid, dim3(G::NT), 0, c->stream, p, c->V, tw, scale);
+    else hipLaunchKernelGGL((k_fft_cols<N, +1, C>), grid, dim3(G::NT), 0, c->stream, p, c->V, tw, scale);
+    return hipGetLastError();
+}
+
+#define JW_SIZE_SWITCH(N, CALL)                         \
+    switch (N) {                                        \
+        case 4: return CALL(4);                         \
+        case 8: return CALL(8);                         \
+        case 16: return CALL(16);                       \
+        case 32: return CALL(32);                       \
+        case 64: return CALL(64);                       \
+        case 128: return CALL(128);                     \
+        case 256: return CALL(256);                     \
+        case 512: return CALL(512);                     \
+        case 1024: return CALL(1024);                   \
+        case 2048: return CALL(2048);                   \
+        case 4096: return CALL(4096);                   \
+        case 8192: return CALL(8192);                   \
+        default: return hipErrorInvalidValue;           \
+    }
+
+template <typename C>
+static const C* twiddles_of(const jw_ctx* c) {
+    if constexpr (sizeof(C) == sizeof(double2)) return c->tw;
+    else return c->twf;
+}
+
+template <typename C>
+static hipError_t fft_rows_t(jw_ctx* c, C* planes, int nplanes, int dir, double scale) {
+#define CALL_ROWS(n) launch_rows<n, C>(c, planes, nplanes, dir, scale, twiddles_of<C>(c))
+    JW_SIZE_SWITCH(c->N, CALL_ROWS)
+#undef CALL_ROWS
+}
+
+template <typename C>
+static hipError_t fft_cols_t(jw_ctx* c, C* planes, int nplanes, int dir, double scale) {
+#define CALL_COLS(n) launch_cols<n, C>(c, planes, nplanes, dir, scale, twiddles_of<C>(c))
+    JW_SIZE_SWITCH(c->N, CALL_COLS)
+#undef CALL_COLS
+}
+
+// unitary 2-D transform of nplanes planes: dir 1 -> exp(-i k.x), dir 0 -> exp(+i k.x),
+// scaled by `scale` once (1/N for the reference's 1/sqrt(V))
+template <typename C>
+static hipError_t fft2d_t(jw_ctx* c, C* planes, int nplanes, int dir, double scale) {
+    hipError_t e = fft_rows_t(c, planes, nplanes, dir, 1.0);
+    if (e != hipSuccess) return e;
+    return fft_cols_t(c, planes, nplanes, dir, scale);
+}
+
+// ===========================================================================
+// generator-basis engine (packed planes)
+//   Xi planes 0..3: (n0,n1),(n2,n3),(n4,n5),(n6,n7) of xi_x; planes 4..7: same for xi_y
+//   Vv planes: the same layout for V_x, V_y;  ZA, ZB: 4 planes (A, B)
+// ===========================================================================
+template <typename C>
+__global__ __launch_bounds__(256) void k_gen_noise_packed(C* __restrict__ Xi, size_t V, NoiseSrc ns,
+                                                          const double* __restrict__ sa_pos) {
+    typedef typename scalar_of<C>::type T;
+    GRID_STRIDE(i, V) {
+        T z[16];
+        normals16_t(ns, i, z);
+        const double s = sa_pos ? sa_pos[i] : 1.0;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            Xi[p * V + i] = mkc<C>(z[4 * p] * s, z[4 * p + 2] * s);           // n_{2p}, n_{2p+1}
+            Xi[(4 + p) * V + i] = mkc<C>(z[4 * p + 1] * s, z[4 * p + 3] * s); // m_{2p}, m_{2p+1}
+        }
+    }
+}
+
+// Z[p] = i (kx(k) X[p] + ky(k) Y[p]),  kx = -2 pi c pbarX / phat2, zero at k = 0;
+// scale_inputs: X, Y *= sa(k) first (and written back) for per-mode correlated noise.
+template <typename C>
+__global__ __launch_bounds__(256) void k_kernel_mult_packed(C* __restrict__ Z, C* __restrict__ X, C* __restrict__ Y,
+                                                            int N, int coupling, const double* __restrict__ q4,
+                                                            const double* __restrict__ sb,
+                                                            const double* __restrict__ sa, int scale_inputs) {
+    typedef typename scalar_of<C>::type T;
+    const size_t V = (size_t)N * N;
+    GRID_STRIDE(i, V) {
+        const int ix = (int)(i / N), iy = (int)(i % N);
+        const double ph = q4[ix] + q4[iy];
+        const double sc = (coupling == 0) ? sa[i] : 1.0;
+        const double kxv = (i == 0) ? 0.0 : -2.0 * M_PI * sc * sb[ix] / ph;
+        const double kyv = (i == 0) ? 0.0 : -2.0 * M_PI * sc * sb[iy] / ph;
+        const double si = scale_inputs ? sa[i] : 1.0;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            C x = X[p * V + i], y = Y[p * V + i];
+            if (scale_inputs) {
+                x = cscale(x, (T)si);
+                y = cscale(y, (T)si);
+                X[p * V + i] = x;
+                Y[p * V + i] = y;
+            }
+            Z[p * V + i] = mkc<C>(-(kxv * x.y + kyv * y.y), kxv * x.x + kyv * y.x);   // i (kx x + ky y)
+        }
+    }
+}
+
+template <typename C>
+__global__ __launch_bounds__(256) void k_uxiu_packed(C* __restrict__ Vv, const C* __restrict__ U,
+                                                     const C* __restrict__ Xi_pos, size_t V, NoiseSrc ns,
+                                                     const double* __restrict__ sa_pos) {
+    typedef typename scalar_of<C>::type T;
+    GRID_STRIDE(i, V) {
+        double hx[8], hy[8];
+        if (Xi_pos) {
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const C a = Xi_pos[p * V + i], b = Xi_pos[(4 + p) * V + i];
+                hx[2 * p] = a.x; hx[2 * p + 1] = a.y;
+                hy[2 * p] = b.x; hy[2 * p + 1] = b.y;
+            }
+        } else {
+            T z[16];
+            normals16_t(ns, i, z);
+            const double s = sa_pos ? sa_pos[i] : 1.0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) { hx[k] = z[2 * k] * s; hy[k] = z[2 * k + 1] * s; }
+        }
+        m3 u;
+#pragma unroll
+        for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
+        double vx[8], vy[8];
+        uxiu_gen(u, hx, vx);
+        uxiu_gen(u, hy, vy);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            Vv[p * V + i] = mkc<C>(vx[2 * p], vx[2 * p + 1]);
+            Vv[(4 + p) * V + i] = mkc<C>(vy[2 * p], vy[2 * p + 1]);
+        }
+    }
+}
+
+template <typename C>
+__global__ __launch_bounds__(256) void k_update_packed(C* __restrict__ U, const C* __restrict__ ZA,
+                                                       const C* __restrict__ ZB, size_t V, double ss, int reunit) {
+    GRID_STRIDE(i, V) {
+        double ha[8], hb[8];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const C a = ZA[p * V + i], b = ZB[p * V + i];
+            ha[2 * p] = a.x; ha[2 * p + 1] = a.y;
+            hb[2 * p] = b.x; hb[2 * p + 1] = b.y;
+        }
+        const m3 eb = exp_i_gen(hb, ss);
+        const m3 ea = exp_i_gen(ha, -ss);
+        m3 u;
+#pragma unroll
+        for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
+        m3 r = mmul(mmul(eb, u), ea);
+        if (reunit) r = su3_project(r);
+#pragma unroll
+        for (int k = 0; k < 9; k++) U[k * V + i] = mkc<C>(r.m[k].x, r.m[k].y);
+    }
+}
+
+// the unfused step in either precision: noise, 2-D FFTs, kernel multiplies,
+// U xi U^dagger, update (reunit: project U onto SU(3) after it)
+template <typename C>
+static hipError_t step_unfused_t(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, const NoiseSrc& ns,
+                                 const double* sa_pos, const double* sa_k, int kc, int corr, double ss, int reunit) {
+    const size_t V = c->V;
+    const double s = 1.0 / c->N;
+    const unsigned g = grid_for(V, 256);
+    hipError_t e;
+#define JW_TRY(x) if ((e = (x)) != hipSuccess) return e
+    {
+        KTimer t(c, K_GEN_NOISE);
+        hipLaunchKernelGGL(k_gen_noise_packed<C>, dim3(g), dim3(256), 0, c->stream, Xi, V, ns, sa_pos);
+        JW_TRY(hipGetLastError());
+    }
+    JW_TRY(fft2d_t(c, Xi, 8, 1, s));
+    {
+        KTimer t(c, K_KMULT);
+        hipLaunchKernelGGL(k_kernel_mult_packed<C>, dim3(g), dim3(256), 0, c->stream, ZA, Xi, Xi + 4 * V, c->N, kc,
+                           c->q4, c->sb, sa_k, corr);
+        JW_TRY(hipGetLastError());
+    }
+    JW_TRY(fft2d_t(c, ZA, 4, 0, s));
+    if (corr) JW_TRY(fft2d_t(c, Xi, 8, 0, s));
+    {
+        KTimer t(c, K_UXIU);
+        hipLaunchKernelGGL(k_uxiu_packed<C>, dim3(g), dim3(256), 0, c->stream, Vv, U, corr ? Xi : nullptr, V, ns,
+                           sa_pos);
+        JW_TRY(hipGetLastError());
+    }
+    JW_TRY(fft2d_t(c, Vv, 8, 1, s));
+    {
+        KTimer t(c, K_KMULT);
+        hipLaunchKernelGGL(k_kernel_mult_packed<C>, dim3(g), dim3(256), 0, c->stream, ZB, Vv, Vv + 4 * V, c->N, kc,
+                           c->q4, c->sb, sa_k, 0);
+        JW_TRY(hipGetLastError());
+    }
+    JW_TRY(fft2d_t(c, ZB, 4, 0, s));
+    {
+        KTimer t(c, K_UPDATE);
+        hipLaunchKernelGGL(k_update_packed<C>, dim3(g), dim3(256), 0, c->stream, U, ZA, ZB, V, ss, reunit);
+        JW_TRY(hipGetLastError());
+    }
+#undef JW_TRY
+    return hipSuccess;
+}
+
+}  // namespace jw

@@ -117,17 +117,60 @@ def test_mixed_projection_keeps_su3(jw):
     c.close()
 
 
-def test_mixed_rejects_unsupported(jw):
-    c = jw.Context(32)
-    U = c.field()
-    with pytest.raises(jw.JwError) as e:
-        c.evolve_mixed(U, 3, 1, 1e-4, None, 1)
-    assert e.value.code == -4
+@pytest.mark.parametrize("N", [32, 64])
+def test_mixed_unfused_passes(jw, N):
+    """The unfused FP32 passes (N outside the fused sizes, or jw_set_fused(0)):
+    one injected step against FP64, and against the fused FP32 passes."""
+    c = jw.Context(N)
+    U0 = mv_start(jw, c, Ny=4)
+    z = rng.normal(size=N * N * 16)
+    U64 = c.field(9, U0)
+    c.inject(z)
+    c.evolve(U64, 4, 1, 5e-5, alpha_for(jw, N, 4), nsteps=1)
+    out = {}
+    for mode in ((c.UNFUSED, c.FUSED) if N >= 64 else (c.FUSED,)):
+        c.set_fused(mode)
+        U = c.field(9, U0)
+        c.inject(z)
+        c.evolve_mixed(U, 4, 1, 5e-5, alpha_for(jw, N, 4), nsteps=1, reunit_every=1)
+        out[mode] = U.download()
+        assert frob_max(out[mode], U64.download()) <= 1e-6, mode
+    if len(out) == 2:
+        assert frob_max(out[c.UNFUSED], out[c.FUSED]) <= 1e-6
+    c.set_fused(c.FUSED)
     c.close()
+
+
+def test_mixed_large_lattice(jw):
+    """N = 4096 (unfused FP32 passes): S(r) after 20 steps against FP64 from the
+    same MV start and Philox stream, compared on the device's correlator."""
+    N = 4096
+    c = jw.Context(N)
+    U0 = c.field()
+    c.philox(3, 0, 0)
+    c.mv_init(U0, 4, 1.0, 0.05, 1.0)
+    al = alpha_for(jw, N, 4)
+    U64, U32 = c.field(), c.field()
+    jw.lib().jw_field_copy(U64.h, U0.h)
+    jw.lib().jw_field_copy(U32.h, U0.h)
+    c.philox(9, 0, 0)
+    c.evolve(U64, 4, 1, 5e-5, al, nsteps=20)
+    c.philox(9, 0, 0)
+    c.evolve_mixed(U32, 4, 1, 5e-5, al, nsteps=20, reunit_every=10)
+    S64 = c.dipole_from_corr(c.measure_corr(U64))
+    S32 = c.dipole_from_corr(c.measure_corr(U32))
+    S0 = c.dipole_from_corr(c.measure_corr(U0))
+    assert np.abs(S64 - S32).max() <= 2e-6
+    assert np.abs(S64 - S0).max() > 1e-6
+    c.close()
+
+
+def test_mixed_rejects_unsupported(jw):
     c = jw.Context(64)
     U = c.field()
-    with pytest.raises(jw.JwError):
+    with pytest.raises(jw.JwError) as e:
         c.evolve_mixed(U, 3, 0, 1e-4, None, 1)     # LINEAR kernel
+    assert e.value.code == -4
     with pytest.raises(jw.JwError):
         c.evolve_mixed(U, 2, 1, 1e-4, None, 1)     # HATTA
     c.close()
