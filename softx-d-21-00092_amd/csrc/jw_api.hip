@@ -117,7 +117,7 @@ int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, jw::c
         jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
         if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
         if (fused) {
-            const jw::StepArgs sa{ns, sa_pos, sa_k, kc == 0, corr, ss, 0, 1};
+            const jw::StepArgs sa{ns, sa_pos, sa_k, kc == 0, corr, ss, 0, 1, it == 0, it < nsteps - 1};
             HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, sa));
             continue;
         }
@@ -220,10 +220,10 @@ int batch_tables(BatchGroup& g, int coupling, const jw::Alpha& al, const double*
 
 // one step of group g on its own stream, noise step from the device counter
 hipError_t batch_group_step(jw_batch* b, BatchGroup& g, int coupling, double step, const double* sa_pos,
-                            const double* sa_k, int reunit, int count, void* U) {
+                            const double* sa_k, int reunit, int count, void* U, int first_noise, int next_noise) {
     jw::NoiseSrc ns{nullptr, b->seed, b->traj0 + (uint32_t)g.first, 0u, JW_TAG_LANGEVIN, b->d_step};
-    const jw::StepArgs a{ns, sa_pos, sa_k, coupling == JW_SQRT_COUPLING_CONSTANT,
-                         coupling == JW_CORRELATED_NOISE_COUPLING, sqrt(step), reunit, count};
+    const jw::StepArgs a{ns,      sa_pos,      sa_k,  coupling == JW_SQRT_COUPLING_CONSTANT,
+                         coupling == JW_CORRELATED_NOISE_COUPLING, sqrt(step), reunit, count, first_noise, next_noise};
     if (b->fp32)
         return jw::step_fused_f32(g.ctx, (jw::cplxf*)U, (jw::cplxf*)g.Xi, (jw::cplxf*)g.Vv, (jw::cplxf*)g.ZA,
                                   (jw::cplxf*)g.ZB, a);
@@ -231,14 +231,14 @@ hipError_t batch_group_step(jw_batch* b, BatchGroup& g, int coupling, double ste
 }
 
 // one step of every group, forked from and joined back to b->stream, then the
-// step counter; capturable
-int batch_step_all(jw_batch* b, int coupling, double step, const jw::Alpha& al, int reunit) {
+// step counter; capturable.  first/next: as StepArgs::first_noise/next_noise.
+int batch_step_all(jw_batch* b, int coupling, double step, const jw::Alpha& al, int reunit, int first, int next) {
     HIP_CHECK(hipEventRecord(b->fork, b->stream));
     for (auto& g : b->g) {
         const double *sa_pos, *sa_k;
         if (int e = batch_tables(g, coupling, al, &sa_pos, &sa_k)) return e;
         HIP_CHECK(hipStreamWaitEvent(g.ctx->stream, b->fork, 0));
-        HIP_CHECK(batch_group_step(b, g, coupling, step, sa_pos, sa_k, reunit, g.count, g.U));
+        HIP_CHECK(batch_group_step(b, g, coupling, step, sa_pos, sa_k, reunit, g.count, g.U, first, next));
         HIP_CHECK(hipEventRecord(g.join, g.ctx->stream));
         HIP_CHECK(hipStreamWaitEvent(b->stream, g.join, 0));
     }
@@ -272,7 +272,7 @@ int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
             float ms = 0.f;
             for (int it = 0; it < 4 && rc == JW_OK; it++) {
                 if (it == 1) hipEventRecord(e0, g.ctx->stream);
-                if (batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, scratch) != hipSuccess)
+                if (batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, scratch, 1, 0) != hipSuccess)
                     rc = fail(JW_ERR_HIP, "tuning step failed");
             }
             hipEventRecord(e1, g.ctx->stream);
@@ -705,7 +705,8 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
         }
         // projection every reunit_every steps and after the last one
         const int reunit = reunit_every > 0 && ((it + 1) % reunit_every == 0 || it == nsteps - 1);
-        const jw::StepArgs sa{ns, sa_pos, sa_k, coupling == JW_SQRT_COUPLING_CONSTANT, corr, sqrt(step), reunit, 1};
+        const jw::StepArgs sa{ns,   sa_pos,     sa_k,   coupling == JW_SQRT_COUPLING_CONSTANT, corr, sqrt(step), reunit, 1,
+                              it == 0, it < nsteps - 1};
         const hipError_t he = fused ? jw::step_fused_f32(c, U32, Xi, Vv, ZA, ZB, sa)
                                     : jw::step_unfused_f32(c, U32, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k,
                                                            coupling == JW_SQRT_COUPLING_CONSTANT ? 0 : 1, corr,
@@ -1035,7 +1036,7 @@ int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw
         HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         int rc = JW_OK;
         for (int it = 0; it < K && rc == JW_OK; it++)
-            rc = batch_step_all(b, coupling, step, al, reunit_every > 0 && it == K - 1);
+            rc = batch_step_all(b, coupling, step, al, reunit_every > 0 && it == K - 1, it == 0, it < K - 1);
         const hipError_t ec = hipStreamEndCapture(b->stream, &graph);
         if (rc) {
             if (graph) hipGraphDestroy(graph);
@@ -1057,7 +1058,9 @@ int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw
         for (; done + K <= nsteps; done += K) HIP_CHECK(hipGraphLaunch(b->graph, b->stream));
     // the rest, launched directly; projected after the last step
     for (int it = done; it < nsteps; it++)
-        if (int e = batch_step_all(b, coupling, step, al, reunit_every > 0 && it == nsteps - 1)) return e;
+        if (int e = batch_step_all(b, coupling, step, al, reunit_every > 0 && it == nsteps - 1, it == done,
+                                   it < nsteps - 1))
+            return e;
     b->step += (uint32_t)nsteps;
     return JW_OK;
 }

@@ -122,6 +122,8 @@ struct StepArgs {
     double sqrt_step;        // sqrt(Delta s)
     int reunit;              // project the updated U onto SU(3)
     int nbatch;              // trajectories per launch (grid z; 1 for a single lattice)
+    int first_noise;         // generate this step's xi first (else the previous update did)
+    int next_noise;          // the update pass also generates the next step's xi
 };
 hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a);
 hipError_t step_fused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a);
