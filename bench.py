@@ -50,15 +50,17 @@ LAMBDA_A = 0.02
 # 182 for per-mode correlated noise, 132 for the white-noise couplings
 REALS_PER_SITE = {"correlated": 182, "white": 132}
 # Algorithmic HBM traffic of each kernel class of the fused step, in complex
-# planes per site per step (read + written; U is 9 planes):
+# planes per launch (read + written; U is 9 planes):
 #   row_noise_fwd  W 8                                        8
-#   col_kmult      A: R 8 W 4 (+W 8 correlated), B: R 8 W 4   24 (32)
+#   col_kmult      A: R 8 W 4 (+W 8 correlated), B: R 8 W 4   12 and 12 (20 and 12)
 #   row_uxiu_fwd   R U 9 (+ xi~ 8 correlated), W 8            17 (25)
 #   row_update     R A 4 + B 4 + U 9, W U 9                   26
-# Correlated: 91 planes = 182 reals per site, the compulsory figure.  White
-# noise: 75 planes = 150 reals (U is read by both row_uxiu_fwd and row_update).
-PLANES = {"correlated": {"row_noise_fwd": 8, "col_kmult": 32, "row_uxiu_fwd": 25, "row_update": 26},
-          "white": {"row_noise_fwd": 8, "col_kmult": 24, "row_uxiu_fwd": 17, "row_update": 26}}
+#                  (FP64: + W 8 when it also writes the next step's xi)
+# Per step: correlated 91 planes = 182 reals per site, the compulsory figure;
+# white noise 75 planes = 150 reals (U is read by both row_uxiu_fwd and
+# row_update).
+PLANES = {"correlated": {"row_noise_fwd": 8, "col_kmult": (20, 12), "row_uxiu_fwd": 25, "row_update": 26},
+          "white": {"row_noise_fwd": 8, "col_kmult": (12, 12), "row_uxiu_fwd": 17, "row_update": 26}}
 # measured HBM traffic per launch of the dominant kernels (rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a plain copy)
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_N2048.json")
@@ -87,15 +89,30 @@ def parse():
 
 
 def kernel_table(kt, nk, V, planes, bytes_per_real):
-    """Per-class time per step, algorithmic GB/s, and the dominant class's roofline."""
-    per_step_ms = {k: ms * n / nk for k, (ms, n) in kt.items() if n}
+    """Per-class time and algorithmic bytes per step (from each class's launch
+    counts), algorithmic GB/s, and the dominant class's roofline."""
+    calls = {k: n for k, (ms, n) in kt.items() if n}
+    n_noise = calls.get("row_noise_fwd", 0)
+    plane_b = 2 * bytes_per_real * V
+    launches_planes = {}
+    if "row_noise_fwd" in calls:
+        launches_planes["row_noise_fwd"] = planes["row_noise_fwd"] * n_noise
+    if "col_kmult" in calls:
+        a_, b_ = planes["col_kmult"]
+        launches_planes["col_kmult"] = (a_ + b_) * calls["col_kmult"] / 2
+    if "row_uxiu_fwd" in calls:
+        launches_planes["row_uxiu_fwd"] = planes["row_uxiu_fwd"] * calls["row_uxiu_fwd"]
+    if "row_update" in calls:
+        u = calls["row_update"]
+        # updates that also wrote the next step's xi: every step whose xi the noise pass did not write
+        launches_planes["row_update"] = planes["row_update"] * u + planes["row_noise_fwd"] * (u - n_noise)
     classes = {}
-    for k, t in per_step_ms.items():
-        if k in planes:
-            b = planes[k] * 2 * bytes_per_real * V
-            classes[k] = {"ms_per_step": round(t, 5), "GBs": round(b / (t * 1e-3) / 1e9, 1),
-                          "launch_ms": round(kt[k][0], 5), "launches_per_step": kt[k][1] / nk,
-                          "alg_bytes_per_step": b}
+    for k, pl in launches_planes.items():
+        ms, n = kt[k]
+        b = pl * plane_b / nk
+        t = ms * n / nk
+        classes[k] = {"ms_per_step": round(t, 5), "GBs": round(b / (t * 1e-3) / 1e9, 1),
+                      "launch_ms": round(ms, 5), "launches_per_step": n / nk, "alg_bytes_per_step": b}
     dom = max(classes, key=lambda k: classes[k]["ms_per_step"]) if classes else None
     roof = None
     if dom:
