@@ -716,10 +716,13 @@ static hipError_t step_fused_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, cons
     const size_t V = c->V;
     const C* tw = twiddles<C>(c);
     hipError_t e;
-    // The next step's noise rides on the update pass in FP64 (417 against 411
-    // steps/s at N = 2048, correlated); in FP32 the separate noise pass runs two
-    // workgroups per CU and stays separate (616 against 583).
-    constexpr bool fuse_noise = sizeof(C) == sizeof(double2);
+    // The next step's noise rides on the update pass in FP64 for N <= 512,
+    // where that measured faster (steps/s, fixed / correlated coupling, fused
+    // against separate: N = 256 10032 / 10783 against 9785 / 10571; 512 7258 /
+    // 7134 against 6670 / 6579); at 1024 and 2048 the separate pass is faster
+    // (1916 / 1742 against 1746 / 1597; 471 / 426 against 463 / 420), and in
+    // FP32 the separate pass, at two workgroups per CU, is faster everywhere.
+    constexpr bool fuse_noise = sizeof(C) == sizeof(double2) && N <= 512;
     const int first_noise = fuse_noise ? a.first_noise : 1;
     const int next_noise = fuse_noise ? a.next_noise : 0;
     if (first_noise) {
