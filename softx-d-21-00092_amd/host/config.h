@@ -10,7 +10,10 @@
 //   seed = 1                Philox key (the reference seeds from the clock)
 //   lambda_L = 6            Lambda*N of the running coupling (field.h:2273)
 //   mu0_over_lambda = 2.5   mu0/Lambda (field.h:2257)
-//   engine = fused          fused | reference (the per-function calls)
+//   engine = fused          fused | reference (the per-function calls) |
+//                           batched (all stat samples evolve together)
+//   precision = fp64        fp64 | mixed (FP32 fields and FFTs, FP64 exponentials)
+//   reunit_every = 10       mixed precision: steps between projections onto SU(3)
 #ifndef JW_HOST_CONFIG_H
 #define JW_HOST_CONFIG_H
 
@@ -35,6 +38,8 @@ enum Coupling {
 
 enum Kernel { LINEAR_KERNEL, SIN_KERNEL };
 
+enum Engine { ENGINE_FUSED, ENGINE_REFERENCE, ENGINE_BATCHED };
+
 class config {
    public:
     int Nx = 64, Ny = 64;
@@ -58,7 +63,9 @@ class config {
 
     double lambda_L = 6.0;
     double mu0_over_lambda = 2.5;
-    bool fused = true;
+    Engine engine = ENGINE_FUSED;
+    bool mixed = false;
+    int reunit_every = 10;
 
     // Returns 1 on success, 0 when the file cannot be read or a required key is
     // missing (the reference returns nothing and reads garbage).
@@ -108,11 +115,30 @@ class config {
             else if (!strcmp(key, "seed")) seed = (unsigned)strtoul(val, nullptr, 10);
             else if (!strcmp(key, "lambda_L")) lambda_L = atof(val);
             else if (!strcmp(key, "mu0_over_lambda")) mu0_over_lambda = atof(val);
-            else if (!strcmp(key, "engine")) fused = strcmp(val, "reference") != 0;
+            else if (!strcmp(key, "engine")) {
+                if (!strcmp(val, "reference")) engine = ENGINE_REFERENCE;
+                else if (!strcmp(val, "batched")) engine = ENGINE_BATCHED;
+                else if (!strcmp(val, "fused")) engine = ENGINE_FUSED;
+                else {
+                    fprintf(stderr, "config: engine %s: expected fused, reference or batched\n", val);
+                    ok = 0;
+                }
+            } else if (!strcmp(key, "precision")) {
+                if (!strcmp(val, "mixed")) mixed = true;
+                else if (!strcmp(val, "fp64")) mixed = false;
+                else {
+                    fprintf(stderr, "config: precision %s: expected fp64 or mixed\n", val);
+                    ok = 0;
+                }
+            } else if (!strcmp(key, "reunit_every")) reunit_every = atoi(val);
             else fprintf(stderr, "config: unknown key %s ignored\n", key);
         }
         fclose(f);
-        return 1;
+        if (engine == ENGINE_REFERENCE && mixed) {
+            fprintf(stderr, "config: the reference engine runs in FP64 only\n");
+            ok = 0;
+        }
+        return ok;
     }
 };
 

@@ -207,13 +207,20 @@ int update_uf(lfield<T, t>* uf, lfield<T, t>* B, lfield<T, t>* A, double step) {
     return 1;
 }
 
-// The whole momentum-space step in the library's fused generator-basis engine
-// (same result as the per-function sequence above, to rounding).
+// nsteps of the momentum-space step in the library's generator-basis engine
+// (same result as the per-function sequence above, to rounding), in FP64 or in
+// the mixed-precision form (config precision = mixed).
 template <class T, int t>
 int evolve(lfield<T, t>* uf, config* cnfg, int nsteps) {
-    jwhost::check(jw_evolve(jwhost::ctx(), uf->f, jwhost::coupling_code(cnfg->CouplingChoice), (int)cnfg->KernelChoice,
-                            cnfg->step, &jwhost::state().alpha, nsteps),
-                  "evolve");
+    const int c = jwhost::coupling_code(cnfg->CouplingChoice);
+    if (cnfg->mixed)
+        jwhost::check(jw_evolve_mixed(jwhost::ctx(), uf->f, c, (int)cnfg->KernelChoice, cnfg->step,
+                                      &jwhost::state().alpha, nsteps, cnfg->reunit_every),
+                      "evolve (mixed precision)");
+    else
+        jwhost::check(jw_evolve(jwhost::ctx(), uf->f, c, (int)cnfg->KernelChoice, cnfg->step, &jwhost::state().alpha,
+                                nsteps),
+                      "evolve");
     return 1;
 }
 

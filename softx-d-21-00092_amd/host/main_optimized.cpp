@@ -6,9 +6,13 @@
 // Per stat sample: MV initial condition (main_optimized.cpp:178-191), then
 // langevin_steps momentum-space Langevin steps (234-311) with the measurement
 // every langevin_steps/measurements steps (317-336), then print (351-355).
-// The Langevin step runs either as the reference's per-function sequence
-// (engine = reference) or as the library's fused generator-basis step
-// (engine = fused, the default).  Noise: Philox stream {seed, stat sample}.
+// The Langevin step runs as the reference's per-function sequence (engine =
+// reference, one step per call), as the library's generator-basis engine
+// (engine = fused, the default; the steps between two measurements in one
+// call), or with all stat samples evolving together in one batch (engine =
+// batched).  precision = mixed selects the mixed-precision form for the
+// latter two.  Noise: Philox stream {seed, stat sample}, the Langevin steps
+// continuing the MV sheets' counter, so every engine draws the same noise.
 //
 // Deviations from the reference, all stated: sum/err are sized by
 // `measurements` (the reference sizes them by langevin_steps,
@@ -73,12 +77,25 @@ int main(int argc, char* argv[]) {
         }
         const int every = cnfg->langevin_steps / cnfg->measurements > 0 ? cnfg->langevin_steps / cnfg->measurements : 1;
 
-        for (int stat = 0; stat < cnfg->stat; stat++) {
-            timespec start;
-            clock_gettime(CLOCK_MONOTONIC, &start);
-            printf("Gathering stat sample no. %i\n", stat);
+        // the reference measures after Langevin step l when l % every == 0 and
+        // l / every < measurements (main_optimized.cpp:317-319)
+        auto measured = [&](int l) { return l % every == 0 && l / every < cnfg->measurements; };
+        // steps from l up to and including the next measured one (or the end)
+        auto chunk = [&](int l) {
+            int n = 1;
+            while (l + n - 1 < cnfg->langevin_steps - 1 && !measured(l + n - 1)) n++;
+            return n;
+        };
+        auto measure = [&](lfield<double, 9>* U, int time) {
+            uf_copy = *U;
+            fourier2->execute2D(&uf_copy, 1);
+            uf_copy.trace(&corr);
+            corr_global.allgather(&corr, mpi);
+            corr_global.average_and_symmetrize();
+            corr_global.reduce(sum[time], err[time], mpi);
+        };
+        auto mv_init = [&](int stat) {
             jwhost::check(jw_noise_philox(jwhost::ctx(), cnfg->seed, (uint32_t)stat, 0), "noise stream");
-
             timespec starti;
             clock_gettime(CLOCK_MONOTONIC, &starti);
             uf.setToUnit();
@@ -92,49 +109,84 @@ int main(int argc, char* argv[]) {
             }
             jw_sync(jwhost::ctx());
             std::cout << "Initial condition time: " << seconds_since(starti) << std::endl;
+        };
 
-            uftmp = uf;
-            for (int langevin = 0; langevin < cnfg->langevin_steps; langevin++) {
+        if (cnfg->engine == ENGINE_BATCHED) {
+            // all stat samples as one batch of trajectories (the library's batched form)
+            timespec start;
+            clock_gettime(CLOCK_MONOTONIC, &start);
+            jw_batch* batch = nullptr;
+            const int bN = Nx, bcount = cnfg->stat;
+            jwhost::check(jw_batch_create(&batch, 0, 1, &bN, &bcount, cnfg->mixed ? 1 : 0), "batch");
+            for (int stat = 0; stat < cnfg->stat; stat++) {
+                printf("Gathering stat sample no. %i\n", stat);
+                mv_init(stat);
+                jwhost::check(jw_batch_set(batch, stat, uf.f), "batch set");
+            }
+            // each sample's stream continues after its MV sheets
+            jwhost::check(jw_batch_philox(batch, cnfg->seed, 0, (uint32_t)MVmodel->Ny_parameter), "batch noise");
+            for (int langevin = 0; langevin < cnfg->langevin_steps;) {
+                const int n = chunk(langevin);
                 timespec starte;
                 clock_gettime(CLOCK_MONOTONIC, &starte);
-                if (cnfg->fused) {
-                    evolve(&uftmp, cnfg, 1);
-                } else {
-                    if (cnfg->CouplingChoice == NOISE_COUPLING_CONSTANT)
-                        generate_gaussian_with_noise_coupling_constant(&xi_local_x, &xi_local_y, momtable, mpi, cnfg);
-                    else
-                        generate_gaussian(&xi_local_x, &xi_local_y, mpi, cnfg);
-                    fourier2->execute2D(&xi_local_x, 1);
-                    fourier2->execute2D(&xi_local_y, 1);
-                    prepare_A_local(&A_local, &xi_local_x, &xi_local_y, momtable, mpi, cnfg->CouplingChoice,
-                                    cnfg->KernelChoice);
-                    fourier2->execute2D(&A_local, 0);
-                    fourier2->execute2D(&xi_local_x, 0);
-                    fourier2->execute2D(&xi_local_y, 0);
-                    uxiulocal(&uxiulocal_x, &uxiulocal_y, &uftmp, &xi_local_x, &xi_local_y);
-                    fourier2->execute2D(&uxiulocal_x, 1);
-                    fourier2->execute2D(&uxiulocal_y, 1);
-                    prepare_A_local(&B_local, &uxiulocal_x, &uxiulocal_y, momtable, mpi, cnfg->CouplingChoice,
-                                    cnfg->KernelChoice);
-                    fourier2->execute2D(&B_local, 0);
-                    update_uf(&uftmp, &B_local, &A_local, cnfg->step);
-                }
-                jw_sync(jwhost::ctx());
-                std::cout << "Evolution time: " << seconds_since(starte) << std::endl;
-
-                if (langevin % every == 0) {
-                    const int time = langevin / every;
-                    if (time < cnfg->measurements) {
-                        uf_copy = uftmp;
-                        fourier2->execute2D(&uf_copy, 1);
-                        uf_copy.trace(&corr);
-                        corr_global.allgather(&corr, mpi);
-                        corr_global.average_and_symmetrize();
-                        corr_global.reduce(sum[time], err[time], mpi);
+                jwhost::check(jw_batch_evolve(batch, jwhost::coupling_code(cnfg->CouplingChoice),
+                                              (int)cnfg->KernelChoice, cnfg->step, &jwhost::state().alpha, n,
+                                              cnfg->reunit_every),
+                              "batch evolve");
+                jwhost::check(jw_batch_sync(batch), "batch evolve");
+                std::cout << "Evolution time: " << seconds_since(starte) << " (" << n << " steps x " << cnfg->stat
+                          << " trajectories)" << std::endl;
+                langevin += n;
+                if (measured(langevin - 1))
+                    for (int stat = 0; stat < cnfg->stat; stat++) {
+                        jwhost::check(jw_batch_get(batch, stat, uftmp.f), "batch get");
+                        measure(&uftmp, (langevin - 1) / every);
                     }
-                }
             }
+            jw_batch_destroy(batch);
             std::cout << "Statistics time: " << seconds_since(start) << std::endl;
+        } else {
+            for (int stat = 0; stat < cnfg->stat; stat++) {
+                timespec start;
+                clock_gettime(CLOCK_MONOTONIC, &start);
+                printf("Gathering stat sample no. %i\n", stat);
+                mv_init(stat);
+                uftmp = uf;
+                for (int langevin = 0; langevin < cnfg->langevin_steps;) {
+                    timespec starte;
+                    clock_gettime(CLOCK_MONOTONIC, &starte);
+                    int n = 1;
+                    if (cnfg->engine == ENGINE_FUSED) {
+                        n = chunk(langevin);
+                        evolve(&uftmp, cnfg, n);
+                    } else {
+                        if (cnfg->CouplingChoice == NOISE_COUPLING_CONSTANT)
+                            generate_gaussian_with_noise_coupling_constant(&xi_local_x, &xi_local_y, momtable, mpi,
+                                                                           cnfg);
+                        else
+                            generate_gaussian(&xi_local_x, &xi_local_y, mpi, cnfg);
+                        fourier2->execute2D(&xi_local_x, 1);
+                        fourier2->execute2D(&xi_local_y, 1);
+                        prepare_A_local(&A_local, &xi_local_x, &xi_local_y, momtable, mpi, cnfg->CouplingChoice,
+                                        cnfg->KernelChoice);
+                        fourier2->execute2D(&A_local, 0);
+                        fourier2->execute2D(&xi_local_x, 0);
+                        fourier2->execute2D(&xi_local_y, 0);
+                        uxiulocal(&uxiulocal_x, &uxiulocal_y, &uftmp, &xi_local_x, &xi_local_y);
+                        fourier2->execute2D(&uxiulocal_x, 1);
+                        fourier2->execute2D(&uxiulocal_y, 1);
+                        prepare_A_local(&B_local, &uxiulocal_x, &uxiulocal_y, momtable, mpi, cnfg->CouplingChoice,
+                                        cnfg->KernelChoice);
+                        fourier2->execute2D(&B_local, 0);
+                        update_uf(&uftmp, &B_local, &A_local, cnfg->step);
+                    }
+                    jw_sync(jwhost::ctx());
+                    std::cout << "Evolution time: " << seconds_since(starte) << std::endl;
+                    langevin += n;
+                    if (measured(langevin - 1)) measure(&uftmp, (langevin - 1) / every);
+                }
+                std::cout << "Statistics time: " << seconds_since(start) << std::endl;
+            }
         }
         for (int i = cnfg->measurements - 1; i < cnfg->measurements; i++)
             print(i, sum[i], err[i], momtable, 1.0 / 3.0 / cnfg->stat, mpi, cnfg->file_name);

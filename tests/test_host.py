@@ -68,9 +68,10 @@ def host_probe(tmp_path_factory):
 int main(int argc, char** argv) {
     config c;
     if (!c.read_config_from_file(argv[1])) return 1;
-    printf("%d %.17g %.17g %d %s %d %d %.17g %d %d %d %d %u %d\n", c.stat, c.mu, c.mass, c.elementaryWilsonLines,
-           c.file_name.c_str(), c.langevin_steps, c.measurements, c.step, (int)c.EvolutionChoice,
-           (int)c.CouplingChoice, (int)c.KernelChoice, c.Nx, c.seed, (int)c.fused);
+    printf("%d %.17g %.17g %d %s %d %d %.17g %d %d %d %d %u %d %d %d\n", c.stat, c.mu, c.mass,
+           c.elementaryWilsonLines, c.file_name.c_str(), c.langevin_steps, c.measurements, c.step,
+           (int)c.EvolutionChoice, (int)c.CouplingChoice, (int)c.KernelChoice, c.Nx, c.seed, (int)c.engine,
+           (int)c.mixed, c.reunit_every);
     mpi_class mpi;
     mpi.mpi_init(&c);
     rand_class r(&mpi, &c);
@@ -102,7 +103,8 @@ def test_config_parser_reference_file(host_probe):
     assert float(f[7]) != 0.0004
     assert int(f[3]) == 50 and f[4] == "output" and int(f[5]) == 100 and int(f[6]) == 100
     assert (int(f[8]), int(f[9]), int(f[10])) == (1, 1, 1)   # MOMENTUM, NOISE, SIN
-    assert int(f[11]) == 64 and int(f[13]) == 1               # defaults: N = 64, fused engine
+    assert int(f[11]) == 64 and int(f[13]) == 0               # defaults: N = 64, fused engine
+    assert int(f[14]) == 0 and int(f[15]) == 10               # FP64; projection every 10 (mixed only)
 
 
 def test_config_parser_extra_keys(host_probe, tmp_path):
@@ -111,7 +113,13 @@ def test_config_parser_extra_keys(host_probe, tmp_path):
     base = base.replace("NOISE_COUPLING_CONSTANT", "CORRELATED_NOISE_COUPLING")
     cfg.write_text(base + "N = 16\nseed = 42\nengine = reference\n")
     f, _, mom = run_probe(host_probe, cfg)
-    assert int(f[9]) == 4 and int(f[11]) == 16 and int(f[12]) == 42 and int(f[13]) == 0
+    assert int(f[9]) == 4 and int(f[11]) == 16 and int(f[12]) == 42 and int(f[13]) == 1
+    cfg.write_text(base + "engine = batched\nprecision = mixed\nreunit_every = 5\n")
+    f, _, _ = run_probe(host_probe, cfg)
+    assert int(f[13]) == 2 and int(f[14]) == 1 and int(f[15]) == 5
+    for bad in ("engine = turbo\n", "precision = fp16\n", "engine = reference\nprecision = mixed\n"):
+        cfg.write_text(base + bad)
+        assert subprocess.run([str(host_probe), str(cfg)], capture_output=True).returncode == 1, bad
 
 
 def test_host_rand_class_and_momenta(host_probe, orc, tmp_path):
