@@ -210,8 +210,8 @@ int jw_saturation_scale(int nbins, const double* rbar, const double* Sbin, const
  * same step.  mode 0: the unfused kernel sequence; 1: the fused row/column
  * passes (default, 64 <= N <= 2048) with the default layout of their
  * intermediate planes; 2, 3, 4: the fused passes with row-major, 4-row-blocked
- * or 8-row-blocked intermediate planes; 5: row-major planes with a column pass
- * built on a register-resident FFT instead of LDS strips (2..5 for
+ * or 8-row-blocked intermediate planes; 5: row-major planes with the column
+ * pass taking one plane at a time over 64-byte column strips (2..5 for
  * measurement). */
 int jw_set_fused(jw_ctx* ctx, int mode);
 /* Enable per-kernel timing with HIP events on the context's stream. */

@@ -268,49 +268,6 @@ __device__ __forceinline__ void dft_small(C* v) {
         v[2] = csub(t0, t2);
         v[1] = cadd(t1, t3);
         v[3] = csub(t1, t3);
-    } else if constexpr (R == 16) {
-        // 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2.  DFT4 over n1 for each n2, twiddle
-        // by w16^(n2 k1), DFT4 over n2 for each k1.
-        const T c1 = (T)0.92387953251128675612818318939679;   // cos(pi/8)
-        const T s1 = (T)0.38268343236508977172845998403040;   // sin(pi/8)
-        const T h = (T)0.70710678118654752440084436210485;
-        C a[4][4];
-#pragma unroll
-        for (int n2 = 0; n2 < 4; n2++) {
-#pragma unroll
-            for (int n1 = 0; n1 < 4; n1++) a[n2][n1] = v[4 * n1 + n2];
-            dft_small<4, DIR>(a[n2]);          // a[n2][k1]
-        }
-        // w16^m for m = n2 k1, in the direction DIR: (cos(2 pi m/16), DIR sin(2 pi m/16))
-        auto rot = [&](C x, int m) -> C {
-            T cr, si;
-            switch (m) {
-                case 1: cr = c1; si = s1; break;
-                case 2: cr = h; si = h; break;
-                case 3: cr = s1; si = c1; break;
-                case 4: cr = 0; si = 1; break;
-                case 6: cr = -h; si = h; break;
-                case 9: cr = -c1; si = -s1; break;
-                default: return x;
-            }
-            if (m == 4) return rot_q<DIR>(x);
-            const T sd = DIR < 0 ? -si : si;
-            C r;
-            r.x = x.x * cr - x.y * sd;
-            r.y = x.x * sd + x.y * cr;
-            return r;
-        };
-#pragma unroll
-        for (int n2 = 1; n2 < 4; n2++)
-#pragma unroll
-            for (int k1 = 1; k1 < 4; k1++) a[n2][k1] = rot(a[n2][k1], n2 * k1);
-#pragma unroll
-        for (int k1 = 0; k1 < 4; k1++) {
-            C b[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
-            dft_small<4, DIR>(b);
-#pragma unroll
-            for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = b[k2];
-        }
     } else {   // R == 8: two radix-4 halves and the e^{DIR i pi k/4} twiddles
         C e[4] = {v[0], v[2], v[4], v[6]};
         C o[4] = {v[1], v[3], v[5], v[7]};
@@ -426,82 +383,6 @@ struct fft_stages<N, N, NSEQ, ES, SS, NT, DIR, PS> {
 template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS = 0, typename C>
 __device__ __forceinline__ void lds_fft(C* buf, const C* __restrict__ tw, int tid) {
     fft_stages<N, 1, NSEQ, ES, SS, NT, DIR, PS>::run(buf, tw, tid);
-}
-
-// ---------------------------------------------------------------------------
-// Register-resident FFT of length N with E = 8 or 16 elements per thread:
-// T = N / E threads per sequence, thread t holds v[m] = x[t + T m] (m < E) on
-// entry and the transform X[t + T m] on exit.  Stages (Stockham, as lds_fft):
-// the first, radix E, works on the registers and writes to LDS; the middle
-// ones read and write LDS; the last reads LDS into the registers.  With the
-// first radix equal to the elements per thread, the output distribution equals
-// the input one, so a forward transform, a per-mode multiply and an inverse
-// transform chain in registers.  buf: this sequence's LDS, padded one slot
-// every E elements (padded<N, log2 E>); barriers are workgroup-wide, so every
-// thread of the workgroup calls with the same N.  Radices: E, then E while
-// the rest is a multiple of E, then the rest (E = 16: N = 2048 -> 16 16 8;
-// E = 8: N = 2048 -> 8 8 8 4, 1024 -> 8 8 8 2, 512 -> 8 8 8, 64 -> 8 8).
-// ---------------------------------------------------------------------------
-template <int E> struct log2_of { static constexpr int value = E == 16 ? 4 : (E == 8 ? 3 : (E == 4 ? 2 : 1)); };
-
-template <int REM, int E>
-struct reg_radix { static constexpr int value = (REM % E == 0) ? E : REM; };
-
-template <int N, int E, int P, int DIR, typename C>
-struct reg_stages {
-    static __device__ __forceinline__ void run(C v[E], C* buf, const C* __restrict__ tw, int t) {
-        constexpr int PS = log2_of<E>::value;
-        constexpr int T = N / E;
-        constexpr int R = reg_radix<N / P, E>::value;
-        constexpr int B = N / R;              // butterflies
-        constexpr int Q = B / T;              // butterflies per thread
-        constexpr bool LAST = (P * R == N);
-        static_assert(R >= 2 && R <= E && Q * T == B && Q * R == E, "register FFT plan");
-        C u[Q][R];
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int j = t + T * q;
-#pragma unroll
-            for (int r = 0; r < R; r++) u[q][r] = buf[lpos<1, 0, PS>(j + B * r, 0)];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int j = t + T * q;
-            const int k = j % P;
-            constexpr int STRIDE = N / (P * R);
-#pragma unroll
-            for (int r = 1; r < R; r++) {
-                C w = tw[r * k * STRIDE];
-                if (DIR > 0) w.y = -w.y;
-                u[q][r] = cmul(u[q][r], w);
-            }
-            dft_small<R, DIR>(u[q]);
-            if constexpr (LAST) {
-#pragma unroll
-                for (int r = 0; r < R; r++) v[q + Q * r] = u[q][r];     // X[j + P r] = X[t + T (q + Q r)]
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; r++) buf[lpos<1, 0, PS>((j - k) * R + k + P * r, 0)] = u[q][r];
-            }
-        }
-        if constexpr (!LAST) {
-            __syncthreads();
-            reg_stages<N, E, P * R, DIR, C>::run(v, buf, tw, t);
-        }
-    }
-};
-
-template <int N, int E, int DIR, typename C>
-__device__ __forceinline__ void reg_fft(C v[E], C* buf, const C* __restrict__ tw, int t) {
-    static_assert((E == 8 || E == 16) && N % E == 0 && N >= 2 * E, "register FFT plan");
-    constexpr int PS = log2_of<E>::value;
-    dft_small<E, DIR>(v);                   // stage 1: butterfly t, inputs x[t + T r]
-#pragma unroll
-    for (int r = 0; r < E; r++) buf[lpos<1, 0, PS>(t * E + r, 0)] = v[r];
-    __syncthreads();
-    reg_stages<N, E, E, DIR, C>::run(v, buf, tw, t);
-    __syncthreads();                        // buf free for the caller
 }
 
 }  // namespace jw

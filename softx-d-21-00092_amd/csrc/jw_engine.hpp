@@ -387,91 +387,6 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
 }
 
 // ---------------------------------------------------------------------------
-// column pass on row-major planes with the register FFT (reg_fft, 8 elements
-// per thread; layout 1): one workgroup per strip of W columns and plane pair
-// p, one plane at a time in LDS.  It measured slower than the LDS-strip pass
-// at N = 2048 (FP64 correlated 1.82 against 1.12 ms per step, FP32 even): the
-// row-major column passes are bound by their 32-byte row segments, not by
-// the FFT.  Thread (c, t) holds column c's modes
-// kx = t + (N/8) m, m < 8:
-//   X^ = F_col X[p] -> acc = i fx X^   (CORR: X^ *= sa, X[p] <- F_col^-1[X^ / V])
-//   Y^ = F_col Y[p] -> acc += i fy Y^  (CORR: likewise for Y[p])
-//   Z[p] = F_col^-1 acc
-// W = 2 (FP64) or 4 (FP32) adjacent columns: 32-byte row segments, with
-// strips sharing cache lines on one XCD (xcd_strip); more at small N so that
-// a workgroup has at least 64 threads.  sa16 holds sa(k) in 16-row blocks.
-// ---------------------------------------------------------------------------
-template <int N, typename C>
-struct ColR {
-    static constexpr int E = 8;                                       // elements per thread
-    static constexpr int TS = N / E;                                  // threads per column
-    static constexpr int W0 = sizeof(C) == sizeof(double2) ? 2 : 4;
-    static constexpr int WMIN = 64 / TS > 0 ? 64 / TS : 1;
-    static constexpr int W = W0 > WMIN ? W0 : WMIN;
-    static constexpr int NT = W * TS;
-    static constexpr int SS = padded<N, log2_of<E>::value>::value + 4;   // sequence stride in LDS
-    static constexpr int WAVES = 4;                                   // per SIMD (128 VGPRs)
-    static_assert(W <= N && N % W == 0 && NT <= 1024, "register column tiling");
-};
-
-template <int N, int CORR, typename C>
-__global__ __launch_bounds__((ColR<N, C>::NT), (ColR<N, C>::WAVES)) void k_col_reg(
-    C* __restrict__ Z, C* __restrict__ X, C* __restrict__ Y, size_t V, int sqrt_coupling,
-    const double* __restrict__ q4, const double* __restrict__ sb, const double* __restrict__ sa16,
-    const C* __restrict__ tw) {
-    typedef typename scalar_of<C>::type T;
-    constexpr int W = ColR<N, C>::W, TS = ColR<N, C>::TS, SS = ColR<N, C>::SS, E = ColR<N, C>::E;
-    __shared__ C lds[W * SS];
-    Z += zofs(4, V);
-    X += zofs(8, V);
-    Y += zofs(8, V);
-    const int tid = threadIdx.x;
-    const int c = tid % W, t = tid / W;
-    const int ky = xcd_strip(blockIdx.x, N / W) * W + c;
-    const size_t po = (size_t)blockIdx.y * V + ky;
-    C* buf = lds + c * SS;
-    const double invV = 1.0 / ((double)N * (double)N);
-    C acc[E];
-    auto direction = [&](C* __restrict__ P, int d) {
-        C v[E];
-        const int tt = opaque_v(t);
-#pragma unroll
-        for (int m = 0; m < E; m++) v[m] = P[po + (size_t)(tt + TS * m) * N];
-        reg_fft<N, E, -1>(v, buf, opaque(tw), tt);
-        const double* q4d = opaque(q4);
-        const double* sbd = opaque(sb);
-        const double* sad = opaque(sa16);
-#pragma unroll
-        for (int m = 0; m < E; m++) {
-            const int kx = tt + TS * m;
-            const size_t k16 = ((size_t)(kx >> 4) * N + ky) * 16 + (kx & 15);
-            if (CORR) v[m] = cscale(v[m], (T)sad[k16]);
-            double f = 0.0;
-            if (kx != 0 || ky != 0) {
-                const double ph = q4d[kx] + q4d[ky];
-                const double cc = sqrt_coupling ? sad[k16] : 1.0;
-                f = -2.0 * M_PI * cc * sbd[d ? ky : kx] / ph * invV;
-            }
-            if (d == 0) acc[m] = mkc<C>(-(f * v[m].y), f * v[m].x);
-            else acc[m] = mkc<C>(acc[m].x - f * v[m].y, acc[m].y + f * v[m].x);
-        }
-        if (CORR) {
-#pragma unroll
-            for (int m = 0; m < E; m++) v[m] = cscale(v[m], (T)invV);
-            reg_fft<N, E, +1>(v, buf, opaque(tw), tt);
-#pragma unroll
-            for (int m = 0; m < E; m++) P[po + (size_t)(tt + TS * m) * N] = v[m];
-        }
-    };
-    direction(X, 0);
-    direction(Y, 1);
-    const int tt = opaque_v(t);
-    reg_fft<N, E, +1>(acc, buf, opaque(tw), tt);
-#pragma unroll
-    for (int m = 0; m < E; m++) Z[po + (size_t)(tt + TS * m) * N] = acc[m];
-}
-
-// ---------------------------------------------------------------------------
 // column pass on the blocked layout: one workgroup per 128-byte column segment
 // (CPB = 8 / BF adjacent columns) and plane pair p, one plane at a time in LDS:
 //   X^ = F_col X[p]  ->  acc = i fx X^       (CORR: X^ *= sa, X[p] <- F_col^-1[X^ / V])
@@ -483,12 +398,16 @@ __global__ __launch_bounds__((ColR<N, C>::NT), (ColR<N, C>::WAVES)) void k_col_r
 // ---------------------------------------------------------------------------
 template <int N, int BF, typename C>
 struct ColB {
-    static constexpr int SEG = 128 / (int)sizeof(C);   // elements per 128-byte segment
+    // BF = 4, 8: one 128-byte segment (SEG elements) holds CPB columns of BF rows;
+    // BF = 1 (row-major): 64-byte row segments of CPB adjacent columns
+    static constexpr int SEG = BF == 1 ? 64 / (int)sizeof(C) : 128 / (int)sizeof(C);
     static constexpr int CPB = SEG / BF;               // columns per workgroup
     static constexpr int ELEMS = CPB * N;
-    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > 512) ? 512 : ELEMS / 8);
+    static constexpr int NTMAX = BF == 1 ? 1024 : 512;
+    static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > NTMAX) ? NTMAX : ELEMS / 8);
     static constexpr int PER = ELEMS / NT;             // elements per thread
-    static_assert((BF == 4 || BF == 8) && CPB >= 1 && PER * NT == ELEMS && N % 16 == 0 && N % CPB == 0,
+    static_assert((BF == 1 || BF == 4 || BF == 8) && CPB >= 1 && CPB <= N && PER * NT == ELEMS && N % 16 == 0 &&
+                      N % CPB == 0,
                   "blocked column tiling");
 };
 
@@ -511,7 +430,8 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __res
     const int y0 = blockIdx.x * CPB;
     const double invV = 1.0 / ((double)N * (double)N);
     // global offset within a plane / LDS slot of loaded element e: x-block
-    // e / SEG, slot e mod SEG = c BF + x mod BF
+    // e / SEG, slot e mod SEG = c BF + x mod BF (row-major, BF = 1: row e / SEG,
+    // column y0 + e mod SEG)
     auto goff = [&](int e) -> size_t { return (size_t)(e / SEG) * ((size_t)N * BF) + (size_t)y0 * BF + (e % SEG); };
     auto lofs = [&](int e) -> int {
         const int sl = e % SEG;
@@ -690,14 +610,6 @@ static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int 
         else
             hipLaunchKernelGGL((k_col_kmult<N, 0, C>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 0>::NT), 0, c->stream, Z, X,
                                Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
-    } else if constexpr (BF == 1) {
-        constexpr int W = ColR<N, C>::W, NT = ColR<N, C>::NT;
-        if (corr)
-            hipLaunchKernelGGL((k_col_reg<N, 1, C>), dim3(N / W, 4, nbatch), dim3(NT), 0, c->stream, Z, X, Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
-        else
-            hipLaunchKernelGGL((k_col_reg<N, 0, C>), dim3(N / W, 4, nbatch), dim3(NT), 0, c->stream, Z, X, Y, V,
                                sqrt_coupling, c->q4, c->sb, sa_k, tw);
     } else {
         constexpr int CPB = ColB<N, BF, C>::CPB, NT = ColB<N, BF, C>::NT;

@@ -69,8 +69,9 @@ struct jw_ctx {
     size_t ws_planes;
     // 1: evolve through the unfused kernel sequence even where the fused passes exist
     int unfused;
-    // fused step's intermediate-plane layout: 0 row-major (LDS-strip column
-    // pass), 1 row-major (register-FFT column pass), 4 or 8 rows per block, -1 default
+    // fused step's intermediate-plane layout: 0 row-major (X and Y tiles of
+    // 32-byte column strips in LDS), 1 row-major (one plane at a time over
+    // 64-byte strips), 4 or 8 rows per block, -1 default
     int layout;
     // per-kernel timing (HIP events on ctx->stream)
     int timing;

@@ -253,7 +253,7 @@ class Context:
         _check(lib().jw_calibration_copy(self.h, nplanes))
 
     # jw_set_fused modes
-    UNFUSED, FUSED, FUSED_ROWMAJOR, FUSED_BLOCK4, FUSED_BLOCK8, FUSED_ROWMAJOR_REG = range(6)
+    UNFUSED, FUSED, FUSED_ROWMAJOR, FUSED_BLOCK4, FUSED_BLOCK8, FUSED_ROWMAJOR_WIDE = range(6)
 
     def set_fused(self, mode=1):
         """mode: False/0 unfused kernel sequence, True/1 fused (default layout),
