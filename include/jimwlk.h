@@ -165,7 +165,8 @@ int jw_evolve_mixed(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double s
  * Trajectory t draws the Philox stream of key {seed, traj0 + t}, so it is the
  * trajectory a single lattice evolves after jw_noise_philox(seed, traj0 + t,
  * step).  The intermediate-plane layout of each group is chosen on first use
- * by timing all three on a copy of its state (jw_batch_set_layout overrides). */
+ * by timing each of layouts 0, 2, 4 and 8 on a copy of its state
+ * (jw_batch_set_layout overrides). */
 typedef struct jw_batch jw_batch;
 int jw_batch_create(jw_batch** out, int device, int ngroups, const int* N, const int* count, int fp32);
 void jw_batch_destroy(jw_batch* b);
@@ -175,7 +176,7 @@ int jw_batch_set(jw_batch* b, int t, const jw_field* U);
 int jw_batch_get(jw_batch* b, int t, jw_field* U);
 /* noise key and step of every trajectory (trajectory t: key {seed, traj0 + t}) */
 int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step);
-/* layout 0, 4 or 8 for group g, or -1 to tune on the next evolution; the chosen one */
+/* layout 0, 2, 4 or 8 for group g, or -1 to tune on the next evolution; the chosen one */
 int jw_batch_set_layout(jw_batch* b, int group, int layout);
 int jw_batch_layout(const jw_batch* b, int group);
 /* nsteps steps of every trajectory (SIN kernel); reunit_every as jw_evolve_mixed
@@ -211,8 +212,9 @@ int jw_saturation_scale(int nbins, const double* rbar, const double* Sbin, const
  * passes (default, 64 <= N <= 2048) with the default layout of their
  * intermediate planes; 2, 3, 4: the fused passes with row-major, 4-row-blocked
  * or 8-row-blocked intermediate planes; 5: row-major planes with the column
- * pass taking one plane at a time over 64-byte column strips (2..5 for
- * measurement). */
+ * pass taking one plane at a time over 64-byte column strips; 6: the two
+ * directions of each plane pair interleaved site by site, so that a column
+ * strip of both is one row segment (2..6 for measurement). */
 int jw_set_fused(jw_ctx* ctx, int mode);
 /* Enable per-kernel timing with HIP events on the context's stream. */
 int jw_timing(jw_ctx* ctx, int enable);

@@ -100,6 +100,9 @@ int check_field(const jw_field* f, const jw_ctx* c, int t, const char* name) {
 // N = 2048 (steps/s, FP64 / FP32 storage, correlated / fixed coupling):
 // row-major 382 / 568, 418 / 653; 4-row blocks 363 / 474, 415 / 536; 8-row
 // blocks 345 / 507, 407 / 559.  Row-major everywhere.
+// the column passes of layouts 1, 4 and 8 read the 16-row-blocked sqrt(alpha) table
+bool uses_sa16(int layout) { return layout == 1 || layout == 4 || layout == 8; }
+
 int default_layout(int N, int corr, int fp32) {
     (void)N;
     (void)corr;
@@ -208,7 +211,7 @@ int batch_tables(BatchGroup& g, int coupling, const jw::Alpha& al, const double*
         *sa_pos = c->sa;
     } else if (coupling == JW_SQRT_COUPLING_CONSTANT || coupling == JW_CORRELATED_NOISE_COUPLING) {
         if (int e = ensure_sa(c, 0, al)) return e;
-        if (c->layout != 0) {
+        if (uses_sa16(c->layout)) {
             if (int e = ensure_sa16(c)) return e;
             *sa_k = c->sa16;
         } else {
@@ -265,7 +268,7 @@ int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
         float best = 1e30f;
         int best_layout = 0;
         int rc = JW_OK;
-        for (int lay : {0, 4, 8}) {
+        for (int lay : {0, 2, 4, 8}) {
             g.ctx->layout = lay;
             const double *sa_pos, *sa_k;
             if ((rc = batch_tables(g, coupling, al, &sa_pos, &sa_k))) break;
@@ -648,7 +651,7 @@ int jw_evolve(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, con
     const bool fused = jw::fused_supported(c->N) && !c->unfused;
     const int saved_layout = c->layout;
     if (c->layout < 0) c->layout = default_layout(c->N, corr, 0);
-    if (fused && sa_k && c->layout != 0) {
+    if (fused && sa_k && uses_sa16(c->layout)) {
         if (int e = ensure_sa16(c)) { c->layout = saved_layout; return e; }
         sa_k = c->sa16;
     }
@@ -691,7 +694,7 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
     const int saved_layout = c->layout;
     if (c->layout < 0) c->layout = default_layout(c->N, corr, 1);
     int rc = JW_OK;
-    if (fused && sa_k && c->layout != 0) {
+    if (fused && sa_k && uses_sa16(c->layout)) {
         rc = ensure_sa16(c);
         sa_k = c->sa16;
     }
@@ -823,8 +826,8 @@ static const char* kNames[] = {"fft_rows", "fft_cols", "gen_noise", "kernel_mult
 
 int jw_set_fused(jw_ctx* c, int mode) {
     if (!c) return fail(JW_ERR_ARG, "null context");
-    static const int layouts[6] = {0, JW_DEFAULT_LAYOUT, 0, 4, 8, 1};
-    if (mode < 0 || mode > 5) return fail(JW_ERR_ARG, "fused mode %d: expected 0..5", mode);
+    static const int layouts[7] = {0, JW_DEFAULT_LAYOUT, 0, 4, 8, 1, 2};
+    if (mode < 0 || mode > 6) return fail(JW_ERR_ARG, "fused mode %d: expected 0..6", mode);
     c->unfused = mode ? 0 : 1;
     c->layout = layouts[mode];
     return JW_OK;
@@ -986,7 +989,8 @@ int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step) {
 
 int jw_batch_set_layout(jw_batch* b, int group, int layout) {
     if (!b || group < 0 || group >= (int)b->g.size()) return fail(JW_ERR_ARG, "bad batch group");
-    if (layout != -1 && layout != 0 && layout != 4 && layout != 8) return fail(JW_ERR_ARG, "layout %d", layout);
+    if (layout != -1 && layout != 0 && layout != 2 && layout != 4 && layout != 8)
+        return fail(JW_ERR_ARG, "layout %d", layout);
     HIP_CHECK(hipStreamSynchronize(b->stream));
     b->g[group].layout = layout;
     b->g[group].ctx->layout = layout;

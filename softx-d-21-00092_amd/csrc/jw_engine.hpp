@@ -69,11 +69,23 @@ __device__ __forceinline__ int xcd_strip(int b, int nstrips) {
     return (b % 8) * (nstrips / 8) + b / 8;
 }
 
-// offset of site (x, y) in an intermediate plane of layout BF
+// offset of site (x, y) in an intermediate plane of layout BF (0 and 2
+// row-major, 1 row-major for the 64-byte-strip column pass, 4 and 8 row blocks)
 template <int N, int BF>
 __device__ __forceinline__ size_t lay(int x, int y) {
-    if constexpr (BF == 0) return (size_t)x * N + y;
+    if constexpr (BF <= 2) return (size_t)x * N + y;
     else return ((size_t)(x / BF) * N + y) * BF + (x % BF);
+}
+
+// offset of site (x, y) of plane q (0..3: the x direction's plane pairs,
+// 4..7: the y direction's) in an 8-plane spectral array.  Layout 2 interleaves
+// the two directions of a plane pair site by site, (p, x, y, d) at
+// 2 V p + 2 (x N + y) + d, so that a column strip of W columns of both
+// directions is one 2W-element row segment.
+template <int N, int BF>
+__device__ __forceinline__ size_t pat(int q, int x, int y, size_t V) {
+    if constexpr (BF == 2) return (size_t)(q & 3) * 2 * V + 2 * ((size_t)x * N + y) + (q >> 2);
+    else return (size_t)q * V + lay<N, BF>(x, y);
 }
 
 // lattice row of row-pass block b.  Blocks are dealt round-robin to the 8 XCDs;
@@ -81,7 +93,7 @@ __device__ __forceinline__ size_t lay(int x, int y) {
 // consecutive turns (a bijection on [0, N) for N a multiple of 8 BF).
 template <int N, int BF>
 __device__ __forceinline__ int row_of_block(int b) {
-    if constexpr (BF == 0) return b;
+    if constexpr (BF <= 2) return b;
     else {
         static_assert(N % (8 * BF) == 0, "blocked layout needs N a multiple of 8 BF");
         const int t = b >> 3, xcd = b & 7;
@@ -147,7 +159,7 @@ __device__ __forceinline__ void row_noise_body(C* lds, C* __restrict__ Xi, size_
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Xi[p * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+        for (int q = 0; q < SPT; q++) Xi[pat<N, BF>(p, x, tid + q * NT, V)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < SPT; q++)
@@ -158,7 +170,7 @@ __device__ __forceinline__ void row_noise_body(C* lds, C* __restrict__ Xi, size_
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Xi[(4 + p) * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+        for (int q = 0; q < SPT; q++) Xi[pat<N, BF>(4 + p, x, tid + q * NT, V)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
 }
 
 template <int N, int BF, typename C>
@@ -215,7 +227,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(C* __restrict__ Vv
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
-        for (int q = 0; q < SPT; q++) Vv[p * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+        for (int q = 0; q < SPT; q++) Vv[pat<N, BF>(p, x, tid + q * NT, V)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < SPT; q++)
@@ -227,7 +239,7 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(C* __restrict__ Vv
     for (int p = 0; p < 4; p++)
 #pragma unroll
         for (int q = 0; q < SPT; q++)
-            Vv[(4 + p) * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+            Vv[pat<N, BF>(4 + p, x, tid + q * NT, V)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
 }
 
 // ---------------------------------------------------------------------------
@@ -257,7 +269,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
         for (int p = 0; p < 4; p++)
 #pragma unroll
             for (int q = 0; q < SPT; q++)
-                lds[lpos<1, PN, 4>(tid + q * NT, p)] = Xs[(4 * half + p) * V + lay<N, BF>(x, tid + q * NT)];
+                lds[lpos<1, PN, 4>(tid + q * NT, p)] = Xs[pat<N, BF>(4 * half + p, x, tid + q * NT, V)];
         __syncthreads();
         lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
         // one site at a time: keeps a single U (9 complex) live
@@ -286,7 +298,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
         for (int p = 0; p < 4; p++)
 #pragma unroll
             for (int q = 0; q < SPT; q++)
-                Vv[(4 * half + p) * V + lay<N, BF>(x, tid + q * NT)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
+                Vv[pat<N, BF>(4 * half + p, x, tid + q * NT, V)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
         __syncthreads();
     }
 }
@@ -296,11 +308,12 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
 //   kx = -2 pi c pbarX / phat2 (c = sa(k) for SQRT, else 1), zero at k = 0.
 //   CORR: X^, Y^ are first multiplied by sa(k) (per-mode correlated noise) and
 //   F_col^-1[sa X^ / V] is written back over X (likewise Y).
-// X, Y: row spectra of plane p = blockIdx.y; strips of W columns.
+// X, Y: row spectra of plane p = blockIdx.y; strips of W columns.  PI: X is
+// the pair-interleaved array (layout 2) holding both directions, Y unused.
 // LDS row x holds [X(x, c0..c0+W-1), Y(x, c0..c0+W-1)]: 2W sequences of length N
 // with element stride 2W.
 // ---------------------------------------------------------------------------
-template <int N, int CORR, typename C>
+template <int N, int CORR, typename C, int PI = 0>
 __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
                                                           C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                           const double* __restrict__ q4,
@@ -320,13 +333,25 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
     Y += zofs(8, V);
     const size_t pofs = (size_t)blockIdx.y * V;
     const double invV = 1.0 / ((double)N * (double)N);
-    // load both tiles: element e -> row e / 2W, slot e % 2W (X for slot < W)
+    // element e of the tile: row e / 2W, LDS slot s (X for s < W, Y above),
+    // global offset g within X (PI: the pair-interleaved array) or Y
+    auto elem = [&](int e, int& x, int& s, size_t& g) {
+        x = e / S2;
+        const int m = e % S2;
+        if constexpr (PI) {
+            s = (m & 1) * W + (m >> 1);
+            g = 2 * pofs + 2 * ((size_t)x * N + col0) + m;
+        } else {
+            s = m;
+            g = pofs + (size_t)x * N + col0 + (m % W);
+        }
+    };
 #pragma unroll
     for (int q = 0; q < PER; q++) {
-        const int e = tid + q * NT;
-        const int x = e / S2, s = e % S2;
-        const size_t g = pofs + (size_t)x * N + col0 + (s % W);
-        lds[lpos<S2, 1, 3>(x, s)] = s < W ? X[g] : Y[g];
+        int x, s;
+        size_t g;
+        elem(tid + q * NT, x, s, g);
+        lds[lpos<S2, 1, 3>(x, s)] = (PI || s < W) ? X[g] : Y[g];
     }
     __syncthreads();
     lds_fft<N, S2, S2, 1, NT, -1, 3>(lds, opaque(tw), opaque_v(tid));
@@ -362,10 +387,10 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
         lds_fft<N, S2, S2, 1, NT, +1, 3>(lds, opaque(tw), opaque_v(tid));
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            const int e = tid + q * NT;
-            const int x = e / S2, s = e % S2;
-            const size_t g = pofs + (size_t)x * N + col0 + (s % W);
-            if (s < W) X[g] = lds[lpos<S2, 1, 3>(x, s)];
+            int x, s;
+            size_t g;
+            elem(tid + q * NT, x, s, g);
+            if (PI || s < W) X[g] = lds[lpos<S2, 1, 3>(x, s)];
             else Y[g] = lds[lpos<S2, 1, 3>(x, s)];
         }
         __syncthreads();
@@ -602,15 +627,14 @@ static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int 
                            int nbatch) {
     const size_t V = c->V;
     const C* tw = twiddles<C>(c);
-    if constexpr (BF == 0) {
-        constexpr int W = ColE<N, C>::W;
+    if constexpr (BF == 0 || BF == 2) {
+        constexpr int W = ColE<N, C>::W, PI = BF == 2;
         if (corr)
-            hipLaunchKernelGGL((k_col_kmult<N, 1, C>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 1>::NT), 0, c->stream, Z, X,
-                               Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
+            hipLaunchKernelGGL((k_col_kmult<N, 1, C, PI>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 1>::NT), 0, c->stream,
+                               Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
         else
-            hipLaunchKernelGGL((k_col_kmult<N, 0, C>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 0>::NT), 0, c->stream, Z, X,
-                               Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
+            hipLaunchKernelGGL((k_col_kmult<N, 0, C, PI>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 0>::NT), 0, c->stream,
+                               Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
     } else {
         constexpr int CPB = ColB<N, BF, C>::CPB, NT = ColB<N, BF, C>::NT;
         if (corr)
@@ -676,6 +700,7 @@ static hipError_t step_fused_layout(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB,
     switch (c->layout) {
         case 0: return step_fused_n<N, 0>(c, U, Xi, Vv, ZA, ZB, a, kcls);
         case 1: return step_fused_n<N, 1>(c, U, Xi, Vv, ZA, ZB, a, kcls);
+        case 2: return step_fused_n<N, 2>(c, U, Xi, Vv, ZA, ZB, a, kcls);
         case 4: return step_fused_n<N, 4>(c, U, Xi, Vv, ZA, ZB, a, kcls);
         case 8: return step_fused_n<N, 8>(c, U, Xi, Vv, ZA, ZB, a, kcls);
         default: return hipErrorInvalidValue;

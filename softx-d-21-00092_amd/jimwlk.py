@@ -253,11 +253,13 @@ class Context:
         _check(lib().jw_calibration_copy(self.h, nplanes))
 
     # jw_set_fused modes
-    UNFUSED, FUSED, FUSED_ROWMAJOR, FUSED_BLOCK4, FUSED_BLOCK8, FUSED_ROWMAJOR_WIDE = range(6)
+    UNFUSED, FUSED, FUSED_ROWMAJOR, FUSED_BLOCK4, FUSED_BLOCK8, FUSED_ROWMAJOR_WIDE, FUSED_INTERLEAVED = range(7)
 
     def set_fused(self, mode=1):
         """mode: False/0 unfused kernel sequence, True/1 fused (default layout),
-        2/3/4 fused with row-major / 4-row / 8-row blocked intermediate planes."""
+        2/3/4 fused with row-major / 4-row / 8-row blocked intermediate planes,
+        5 row-major with the column pass over one plane of 64-byte strips, 6 the
+        two directions of each plane pair interleaved site by site."""
         _check(lib().jw_set_fused(self.h, int(mode)))
 
     def kernel_times(self):

@@ -85,14 +85,16 @@ def test_mixed_layouts_agree(jw, N, coupling):
     U0 = mv_start(jw, c, Ny=2)
     al = alpha_for(jw, N, coupling)
     out = {}
-    for mode in (c.FUSED_ROWMAJOR, c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED_ROWMAJOR_WIDE, c.FUSED):
+    for mode in (c.FUSED_ROWMAJOR, c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED_ROWMAJOR_WIDE, c.FUSED_INTERLEAVED,
+                 c.FUSED):
         c.set_fused(mode)
         U = c.field(9, U0)
         c.philox(8, 0, 0)
         c.evolve_mixed(U, coupling, 1, 1e-4, al, nsteps=3, reunit_every=0)
         out[mode] = U.download()
     ref = out[c.FUSED_ROWMAJOR]
-    for mode in (c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED_ROWMAJOR_WIDE, c.FUSED):
+    for mode in (c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED_ROWMAJOR_WIDE, c.FUSED_INTERLEAVED,
+                 c.FUSED):
         assert frob_max(ref, out[mode]) <= 2e-6, mode
     assert frob_max(ref, U0) > 1e-3
     c.set_fused(c.FUSED)
