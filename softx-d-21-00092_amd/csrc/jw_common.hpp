@@ -3,6 +3,7 @@
 // and the lambda-basis map, SoA 3x3 loads/stores and U xi U^dagger in the
 // generator basis.
 #pragma once
+#include <type_traits>
 
 #include "jw_device.hpp"
 #include "jw_internal.hpp"
@@ -151,23 +152,27 @@ __device__ __forceinline__ void store9(cplx* __restrict__ f, size_t V, size_t i,
 
 // V_i = U xi_i U^dagger in generator coefficients (upper triangle + real diagonal)
 // Computed one row r of W = U X at a time, then the entries V_rc (c >= r) that
-// gen_from_upper reads: keeps few values live.
-__device__ __forceinline__ void uxiu_gen(const m3& u, const double h[8], double out[8]) {
-    const double d8 = h[7] * JW_INV_SQRT3;
-    const double x00 = h[2] + d8, x11 = -h[2] + d8, x22 = -2.0 * d8;
-    const cplx x01 = mk(h[0], h[1]), x02 = mk(h[3], h[4]), x12 = mk(h[5], h[6]);
-    double d[3];
-    cplx up[3];   // V01, V02, V12
+// gen_from_upper reads: keeps few values live.  M = m3 with S = double (FP64
+// form), or m3f with S = float (the mixed form's products).
+template <typename M, typename S>
+__device__ __forceinline__ void uxiu_gen(const M& u, const S h[8], S out[8]) {
+    typedef decltype(u.m[0]) CR;
+    typedef typename std::remove_const<typename std::remove_reference<CR>::type>::type Cx;
+    const S d8 = h[7] * (S)JW_INV_SQRT3;
+    const S x00 = h[2] + d8, x11 = -h[2] + d8, x22 = (S)-2.0 * d8;
+    const Cx x01 = mkc<Cx>(h[0], h[1]), x02 = mkc<Cx>(h[3], h[4]), x12 = mkc<Cx>(h[5], h[6]);
+    S d[3];
+    Cx up[3];   // V01, V02, V12
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        const cplx a = u.m[3 * r], b = u.m[3 * r + 1], c = u.m[3 * r + 2];
+        const Cx a = u.m[3 * r], b = u.m[3 * r + 1], c = u.m[3 * r + 2];
         // row r of U X, X Hermitian: X10 = conj(x01), X20 = conj(x02), X21 = conj(x12)
-        const cplx w0 = cadd(cadd(cscale(a, x00), cmulc(b, x01)), cmulc(c, x02));
-        const cplx w1 = cadd(cadd(cmul(a, x01), cscale(b, x11)), cmulc(c, x12));
-        const cplx w2 = cadd(cadd(cmul(a, x02), cmul(b, x12)), cscale(c, x22));
+        const Cx w0 = cadd(cadd(cscale(a, x00), cmulc(b, x01)), cmulc(c, x02));
+        const Cx w1 = cadd(cadd(cmul(a, x01), cscale(b, x11)), cmulc(c, x12));
+        const Cx w2 = cadd(cadd(cmul(a, x02), cmul(b, x12)), cscale(c, x22));
 #pragma unroll
         for (int cc = r; cc < 3; cc++) {
-            cplx s = cmulc(w0, u.m[3 * cc]);
+            Cx s = cmulc(w0, u.m[3 * cc]);
             s = cadd(s, cmulc(w1, u.m[3 * cc + 1]));
             s = cadd(s, cmulc(w2, u.m[3 * cc + 2]));
             if (cc == r) d[r] = s.x;
@@ -177,8 +182,8 @@ __device__ __forceinline__ void uxiu_gen(const m3& u, const double h[8], double 
     out[0] = up[0].x; out[1] = up[0].y;
     out[3] = up[1].x; out[4] = up[1].y;
     out[5] = up[2].x; out[6] = up[2].y;
-    out[2] = 0.5 * (d[0] - d[1]);
-    out[7] = (d[0] + d[1] - 2.0 * d[2]) * (0.5 * JW_INV_SQRT3);
+    out[2] = (S)0.5 * (d[0] - d[1]);
+    out[7] = (d[0] + d[1] - (S)2.0 * d[2]) * (S)(0.5 * JW_INV_SQRT3);
 }
 
 }  // namespace jw

@@ -37,6 +37,7 @@ __device__ __forceinline__ cplxf cmul(cplxf a, cplxf b) { return mkf(a.x * b.x -
 __device__ __forceinline__ cplxf cscale(cplxf a, float s) { return mkf(a.x * s, a.y * s); }
 __device__ __forceinline__ cplxf cmuli(cplxf a) { return mkf(-a.y, a.x); }
 __device__ __forceinline__ cplxf cmulmi(cplxf a) { return mkf(a.y, -a.x); }
+__device__ __forceinline__ cplxf cmulc(cplxf a, cplxf b) { return mkf(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
 
 // complex value of type C (double2 or float2) from two doubles, and back
 template <typename C>
@@ -63,6 +64,36 @@ __device__ __forceinline__ m3 mmul(const m3& a, const m3& b) {
             s = cadd(s, cmul(a.m[3 * i + 2], b.m[6 + j]));
             c.m[3 * i + j] = s;
         }
+    return c;
+}
+
+// the same in FP32 (the mixed-precision form's products of exponentials and
+// links: its fields carry FP32 rounding anyway)
+struct m3f { cplxf m[9]; };
+
+__device__ __forceinline__ m3f mmul(const m3f& a, const m3f& b) {
+    m3f c;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplxf s = cmul(a.m[3 * i], b.m[j]);
+            s = cadd(s, cmul(a.m[3 * i + 1], b.m[3 + j]));
+            s = cadd(s, cmul(a.m[3 * i + 2], b.m[6 + j]));
+            c.m[3 * i + j] = s;
+        }
+    return c;
+}
+__device__ __forceinline__ m3f to_f(const m3& a) {
+    m3f c;
+#pragma unroll
+    for (int k = 0; k < 9; k++) c.m[k] = mkf((float)a.m[k].x, (float)a.m[k].y);
+    return c;
+}
+__device__ __forceinline__ m3 to_d(const m3f& a) {
+    m3 c;
+#pragma unroll
+    for (int k = 0; k < 9; k++) c.m[k] = make_double2((double)a.m[k].x, (double)a.m[k].y);
     return c;
 }
 
@@ -159,6 +190,69 @@ __device__ __forceinline__ m3 exp_i_gen(const double h[8], double s) {
     E.m[0] = cadd(E.m[0], f0);
     E.m[4] = cadd(E.m[4], f0);
     E.m[8] = cadd(E.m[8], f0);
+    for (int j = 0; j < k; j++) E = mmul(E, E);
+    return E;
+}
+
+// exp(i s H(h)) as exp_i_gen, rounded to FP32 for the mixed-precision form:
+// the same FP64 series coefficients, X and X^2 formed from their upper
+// triangles (both are Hermitian), the squarings (c1 > 1.92 only) in FP32.
+// Fewer live registers than the FP64 matrices of exp_i_gen.
+template <typename H>
+__device__ __forceinline__ m3f exp_i_gen_f(const H h[8], double s) {
+    double c1 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) c1 += (double)h[a] * (double)h[a];
+    c1 *= s * s;
+    m3f E;
+    if (!(c1 > 0.0)) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) E.m[k] = mkf((k % 4 == 0) ? 1.f : 0.f, 0.f);
+        if (c1 != c1) E.m[0] = mkf((float)c1, (float)c1);
+        return E;
+    }
+    int k = 0;
+    while (c1 > JW_EXP_SERIES_C1 && k < 60) { c1 *= 0.25; k++; }
+    const double sk = ldexp(s, -k);
+    // X = [[d0, x, y], [x*, d1, z], [y*, z*, d2]]
+    const double d8 = (double)h[7] * JW_INV_SQRT3;
+    const double d0 = sk * ((double)h[2] + d8), d1 = sk * (-(double)h[2] + d8), d2 = sk * (-2.0 * d8);
+    const cplx x = mk(sk * (double)h[0], sk * (double)h[1]), y = mk(sk * (double)h[3], sk * (double)h[4]),
+               z = mk(sk * (double)h[5], sk * (double)h[6]);
+    const double ax = x.x * x.x + x.y * x.y, ay = y.x * y.x + y.y * y.y, az = z.x * z.x + z.y * z.y;
+    const cplx xz = cmul(x, z);
+    const double c0 = d0 * d1 * d2 + 2.0 * (xz.x * y.x + xz.y * y.y) - d0 * az - d1 * ay - d2 * ax;
+    double A = 1.0, B = 0.0, D = 0.0;
+    double f0r = 1.0, f0i = 0.0, f1r = 0.0, f1i = 0.0, f2r = 0.0, f2i = 0.0;
+#pragma unroll
+    for (int n = 1; n <= JW_EXP_SERIES_TERMS; n++) {
+        const double inv = 1.0 / n;
+        const double An = c0 * D * inv, Bn = (A + c1 * D) * inv, Dn = B * inv;
+        A = An; B = Bn; D = Dn;
+        if ((n & 3) == 0) { f0r += A; f1r += B; f2r += D; }
+        else if ((n & 3) == 1) { f0i += A; f1i += B; f2i += D; }
+        else if ((n & 3) == 2) { f0r -= A; f1r -= B; f2r -= D; }
+        else { f0i -= A; f1i -= B; f2i -= D; }
+    }
+    const cplx f1 = mk(f1r, f1i), f2 = mk(f2r, f2i);
+    // X^2: real diagonal and upper triangle
+    const double q0 = d0 * d0 + ax + ay, q1 = ax + d1 * d1 + az, q2 = ay + az + d2 * d2;
+    const cplx q01 = cadd(cscale(x, d0 + d1), cmulc(y, z));      // x (d0 + d1) + y z*
+    const cplx q02 = cadd(cscale(y, d0 + d2), xz);               // y (d0 + d2) + x z
+    const cplx q12 = cadd(cmulc(y, x), cscale(z, d1 + d2));      // x* y + z (d1 + d2)
+    // E = f0 + f1 X + f2 X^2; the lower triangle from conj(X), conj(X^2)
+    E.m[0] = mkf((float)(f0r + f1r * d0 + f2r * q0), (float)(f0i + f1i * d0 + f2i * q0));
+    E.m[4] = mkf((float)(f0r + f1r * d1 + f2r * q1), (float)(f0i + f1i * d1 + f2i * q1));
+    E.m[8] = mkf((float)(f0r + f1r * d2 + f2r * q2), (float)(f0i + f1i * d2 + f2i * q2));
+    auto off = [&](cplx a, cplx b, cplxf& up, cplxf& lo) {
+        const cplx u = cadd(cmul(f1, a), cmul(f2, b));
+        const cplx l = cadd(cmul(f1, conjg(a)), cmul(f2, conjg(b)));
+        up = mkf((float)u.x, (float)u.y);
+        lo = mkf((float)l.x, (float)l.y);
+    };
+    off(x, q01, E.m[1], E.m[3]);
+    off(y, q02, E.m[2], E.m[6]);
+    off(z, q12, E.m[5], E.m[7]);
     for (int j = 0; j < k; j++) E = mmul(E, E);
     return E;
 }

@@ -112,6 +112,19 @@ __device__ __forceinline__ NoiseSrc batch_noise(NoiseSrc ns, size_t V) {
 }
 __device__ __forceinline__ size_t zofs(int planes, size_t V) { return (size_t)blockIdx.z * planes * V; }
 
+// matrix type of the site algebra's products: FP64 in the FP64 form, FP32 in
+// the mixed form (whose exponentials and projection stay FP64)
+template <typename C>
+struct site_mat { typedef m3 type; };
+template <>
+struct site_mat<float2> { typedef m3f type; };
+
+// ordering fences: values passed through them are final at the fence and
+// values read after it are defined there (empty asm statements keep their order)
+__device__ __forceinline__ void fence(double& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void fence(float& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void fence(float2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
+
 // the same pointer, opaque to the optimiser: loads through it are not merged
 // with earlier loads of the same addresses (which would keep every twiddle of
 // one FFT live in registers until the next)
@@ -125,6 +138,23 @@ __device__ __forceinline__ const T* opaque(const T* p) {
 __device__ __forceinline__ int opaque_v(int v) {
     asm volatile("" : "+v"(v));
     return v;
+}
+
+// inverse row FFT of 4 planes in LDS; in FP32 through opaque twiddles and
+// thread index (fewer live registers around the site algebra)
+template <int N, int NT, int PN, typename C>
+__device__ __forceinline__ void row_ifft4(C* lds, const C* __restrict__ tw, int tid) {
+    if constexpr (sizeof(C) == sizeof(float2)) lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, opaque(tw), opaque_v(tid));
+    else lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
+}
+
+// a thread's site index in a row pass's site loop; in FP32 opaque, so that the
+// compiler runs the sites one after another instead of interleaving their
+// site algebra (registers for two workgroups per CU)
+template <typename C>
+__device__ __forceinline__ int site_of(int y) {
+    if constexpr (sizeof(C) == sizeof(float2)) return opaque_v(y);
+    else return y;
 }
 
 // ---------------------------------------------------------------------------
@@ -187,7 +217,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_noise_f
 // xi regenerated from the noise source (white noise)
 // ---------------------------------------------------------------------------
 template <int N, int BF, typename C>
-__global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(C* __restrict__ Vv, const C* __restrict__ U,
+__global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_fwd(C* __restrict__ Vv, const C* __restrict__ U,
                                                              size_t V, NoiseSrc ns, const double* __restrict__ sa_pos,
                                                              const C* __restrict__ tw) {
     typedef typename scalar_of<C>::type T;
@@ -203,22 +233,26 @@ __global__ __launch_bounds__(RowE<N>::NT) void k_row_uxiu_fwd(C* __restrict__ Vv
     T vy[SPT][8];      // in the element type: they go to LDS in it
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
-        const int y = tid + q * NT;
+        const int y = site_of<C>(tid + q * NT);
         const size_t i = row0 + y;
-        double hx[8], hy[8];
+        T hx[8], hy[8];
         T z[16];
         normals16_t(ns, i, z);
-        const double s = sa_pos ? sa_pos[i] : 1.0;
+        const T s = sa_pos ? (T)sa_pos[i] : (T)1;
 #pragma unroll
         for (int k = 0; k < 8; k++) { hx[k] = z[2 * k] * s; hy[k] = z[2 * k + 1] * s; }
-        m3 u;
+        // FP64 form: FP64 products; mixed form: FP32 products
+        typename site_mat<C>::type u;
 #pragma unroll
-        for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
-        double vx[8], v2[8];
+        for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+        T vx[8], v2[8];
         uxiu_gen(u, hx, vx);
         uxiu_gen(u, hy, v2);
 #pragma unroll
-        for (int k = 0; k < 8; k++) vy[q][k] = (T)v2[k];
+        for (int k = 0; k < 8; k++) {
+            vy[q][k] = (T)v2[k];
+            if constexpr (sizeof(C) == sizeof(float2)) fence(vy[q][k]);
+        }
 #pragma unroll
         for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mkc<C>(vx[2 * p], vx[2 * p + 1]);
     }
@@ -254,6 +288,7 @@ template <int N, int BF, typename C>
 __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_corr(C* __restrict__ Vv, const C* __restrict__ U,
                                                               const C* __restrict__ Xs, size_t V,
                                                               const C* __restrict__ tw) {
+    typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;
     __shared__ C lds[4 * PN];
@@ -277,17 +312,17 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
         for (int q = 0; q < SPT; q++) {
             const int y = tid + q * NT;
             const size_t i = row0 + y;
-            double h[8];
+            T h[8];
 #pragma unroll
             for (int p = 0; p < 4; p++) {
                 const C a = lds[lpos<1, PN, 4>(y, p)];
                 h[2 * p] = a.x;
                 h[2 * p + 1] = a.y;
             }
-            m3 u;
+            typename site_mat<C>::type u;
 #pragma unroll
-            for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
-            double v[8];
+            for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+            T v[8];
             uxiu_gen(u, h, v);
 #pragma unroll
             for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mkc<C>(v[2 * p], v[2 * p + 1]);
@@ -544,7 +579,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __res
 // forms; U is stored in the element type C.
 // ---------------------------------------------------------------------------
 template <int N, int BF, typename C>
-__global__ __launch_bounds__((RowE<N, 4>::NT)) void k_row_update(C* __restrict__ U, const C* __restrict__ ZA,
+__global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_update(C* __restrict__ U, const C* __restrict__ ZA,
                                                               const C* __restrict__ ZB, size_t V, double ss,
                                                               int reunit, const C* __restrict__ tw, int next_noise,
                                                               C* __restrict__ Xi, NoiseSrc ns,
@@ -565,7 +600,7 @@ __global__ __launch_bounds__((RowE<N, 4>::NT)) void k_row_update(C* __restrict__
 #pragma unroll
         for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZA[p * V + lay<N, BF>(x, tid + q * NT)];
     __syncthreads();
-    lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
+    row_ifft4<N, NT, PN>(lds, tw, tid);
     T ha[SPT][8];
 #pragma unroll
     for (int q = 0; q < SPT; q++)
@@ -581,29 +616,51 @@ __global__ __launch_bounds__((RowE<N, 4>::NT)) void k_row_update(C* __restrict__
 #pragma unroll
         for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZB[p * V + lay<N, BF>(x, tid + q * NT)];
     __syncthreads();
-    lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
+    row_ifft4<N, NT, PN>(lds, tw, tid);
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
-        const int y = tid + q * NT;
+        const int y = site_of<C>(tid + q * NT);
         const size_t i = row0 + y;
-        double hb[8];
+        T hb[8];
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const C b = lds[lpos<1, PN, 4>(y, p)];
             hb[2 * p] = b.x;
             hb[2 * p + 1] = b.y;
         }
-        m3 u;
+        if constexpr (sizeof(C) == sizeof(double2)) {
+            double hd[8];
 #pragma unroll
-        for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
-        double hd[8];
+            for (int k = 0; k < 8; k++) hd[k] = ha[q][k];
+            m3 u;
 #pragma unroll
-        for (int k = 0; k < 8; k++) hd[k] = ha[q][k];
-        const m3 w = mmul(exp_i_gen(hb, ss), u);
-        m3 r = mmul(w, exp_i_gen(hd, -ss));
-        if (reunit) r = su3_project(r);
+            for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+            const m3 w = mmul(exp_i_gen(hb, ss), u);
+            m3 r = mmul(w, exp_i_gen(hd, -ss));
+            if (reunit) r = su3_project(r);
 #pragma unroll
-        for (int k = 0; k < 9; k++) U[k * V + i] = mkc<C>(r.m[k].x, r.m[k].y);
+            for (int k = 0; k < 9; k++) U[k * V + i] = r.m[k];
+        } else {
+            // FP64 exponentials, FP32 products; the projection in FP64.  The
+            // fences keep the two exponentials apart (fewer live registers).
+            const m3f eb = exp_i_gen_f(hb, ss);
+            m3f u;
+#pragma unroll
+            for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+            m3f w = mmul(eb, u);
+            T hd[8];
+#pragma unroll
+            for (int k = 0; k < 9; k++) fence(w.m[k]);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                hd[k] = ha[q][k];
+                fence(hd[k]);
+            }
+            m3f r = mmul(w, exp_i_gen_f(hd, -ss));
+            if (reunit) r = to_f(su3_project(to_d(r)));
+#pragma unroll
+            for (int k = 0; k < 9; k++) U[k * V + i] = r.m[k];
+        }
     }
     if (next_noise) {        // the next step's xi row spectra
         __syncthreads();
