@@ -124,6 +124,7 @@ struct site_mat<float2> { typedef m3f type; };
 __device__ __forceinline__ void fence(double& v) { asm volatile("" : "+v"(v)); }
 __device__ __forceinline__ void fence(float& v) { asm volatile("" : "+v"(v)); }
 __device__ __forceinline__ void fence(float2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
+__device__ __forceinline__ void fence(double2& v) { asm volatile("" : "+v"(v.x), "+v"(v.y)); }
 
 // the same pointer, opaque to the optimiser: loads through it are not merged
 // with earlier loads of the same addresses (which would keep every twiddle of
@@ -144,7 +145,7 @@ __device__ __forceinline__ int opaque_v(int v) {
 // thread index (fewer live registers around the site algebra)
 template <int N, int NT, int PN, typename C>
 __device__ __forceinline__ void row_ifft4(C* lds, const C* __restrict__ tw, int tid) {
-    if constexpr (sizeof(C) == sizeof(float2)) lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, opaque(tw), opaque_v(tid));
+    if constexpr (sizeof(C) == sizeof(float2) || N <= 1024) lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, opaque(tw), opaque_v(tid));
     else lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
 }
 
@@ -153,8 +154,7 @@ __device__ __forceinline__ void row_ifft4(C* lds, const C* __restrict__ tw, int 
 // site algebra (registers for two workgroups per CU)
 template <typename C>
 __device__ __forceinline__ int site_of(int y) {
-    if constexpr (sizeof(C) == sizeof(float2)) return opaque_v(y);
-    else return y;
+    return opaque_v(y);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_fw
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             vy[q][k] = (T)v2[k];
-            if constexpr (sizeof(C) == sizeof(float2)) fence(vy[q][k]);
+            fence(vy[q][k]);
         }
 #pragma unroll
         for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mkc<C>(vx[2 * p], vx[2 * p + 1]);
@@ -629,13 +629,18 @@ __global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_up
             hb[2 * p + 1] = b.y;
         }
         if constexpr (sizeof(C) == sizeof(double2)) {
-            double hd[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) hd[k] = ha[q][k];
             m3 u;
 #pragma unroll
             for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
-            const m3 w = mmul(exp_i_gen(hb, ss), u);
+            m3 w = mmul(exp_i_gen(hb, ss), u);
+            double hd[8];
+#pragma unroll
+            for (int k = 0; k < 9; k++) fence(w.m[k]);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                hd[k] = ha[q][k];
+                fence(hd[k]);
+            }
             m3 r = mmul(w, exp_i_gen(hd, -ss));
             if (reunit) r = su3_project(r);
 #pragma unroll
