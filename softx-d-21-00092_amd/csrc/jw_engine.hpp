@@ -392,9 +392,14 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
     lds_fft<N, S2, S2, 1, NT, -1, 3>(lds, opaque(tw), opaque_v(tid));
     // pair (x, c): X^ in slot c of LDS row x, Y^ in slot W + c
     C zreg[CORR ? PAIRS : 1];
+    // FP32 correlated: indices recomputed per phase (opaque_v), since addresses
+    // kept live across the FFTs would cost the registers of a second
+    // workgroup per CU (measured 0.76 -> 0.70 ms per step at N = 2048)
+    constexpr bool REIDX = CORR && sizeof(C) == sizeof(float2);
+    const int tk = REIDX ? opaque_v(tid) : tid;
 #pragma unroll
     for (int q = 0; q < PAIRS; q++) {
-        const int pr = tid + q * NT;
+        const int pr = tk + q * NT;
         const int kx = pr / W, c = pr % W, ky = col0 + c;
         const size_t k = (size_t)kx * N + ky;
         C vx = lds[lpos<S2, 1, 3>(kx, c)], vy = lds[lpos<S2, 1, 3>(kx, W + c)];
@@ -420,27 +425,30 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
     if (CORR) {
         // xi~ = F_col^-1 of the scaled tiles, written back over X and Y
         lds_fft<N, S2, S2, 1, NT, +1, 3>(lds, opaque(tw), opaque_v(tid));
+        const int ts = REIDX ? opaque_v(tid) : tid;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             int x, s;
             size_t g;
-            elem(tid + q * NT, x, s, g);
+            elem(ts + q * NT, x, s, g);
             if (PI || s < W) X[g] = lds[lpos<S2, 1, 3>(x, s)];
             else Y[g] = lds[lpos<S2, 1, 3>(x, s)];
         }
         __syncthreads();
+        const int tz = REIDX ? opaque_v(tid) : tid;
 #pragma unroll
         for (int q = 0; q < PAIRS; q++) {
-            const int pr = tid + q * NT;
+            const int pr = tz + q * NT;
             lds[lpos<S2, 1, 3>(pr / W, pr % W)] = zreg[q];
         }
         __syncthreads();
     }
     // Z: the W sequences in the X slots
     lds_fft<N, W, S2, 1, NT, +1, 3>(lds, opaque(tw), opaque_v(tid));
+    const int to = REIDX ? opaque_v(tid) : tid;
 #pragma unroll
     for (int q = 0; q < PAIRS; q++) {
-        const int pr = tid + q * NT;
+        const int pr = to + q * NT;
         const int x = pr / W, c = pr % W;
         Z[pofs + (size_t)x * N + col0 + c] = lds[lpos<S2, 1, 3>(x, c)];
     }
