@@ -3,11 +3,12 @@ FFTs, FP64 exponentials and SU(3) projection, against the FP64 form on the
 same noise.
 
 Tolerances:
-  * one step from the same U and injected noise: per-site Frobenius
-    <= 1e-6 (FP32 storage of U alone is ~1e-7);
+  * one step from the same U and injected noise, or the same Philox stream
+    (FP32 Box-Muller from the same Philox words): per-site Frobenius <= 1e-6
+    (FP32 storage of U alone is ~1e-7);
   * 1000 steps from the same MV start and Philox stream: dipole S(r) within
     2e-5 absolute of the FP64 form (the acceptance bound of the mixed form);
-  * the four intermediate layouts agree with each other to FP32 rounding;
+  * the intermediate layouts agree with each other to FP32 rounding;
   * with projection, U is unitary with unit determinant to FP32 rounding.
 """
 import math
@@ -50,6 +51,25 @@ def test_mixed_one_step_injected(jw, coupling):
     c.inject(z)
     c.evolve(U64, coupling, 1, ds, al, nsteps=1)
     c.inject(z)
+    c.evolve_mixed(U32, coupling, 1, ds, al, nsteps=1, reunit_every=0)
+    a, b = U64.download(), U32.download()
+    assert frob_max(a, b) <= 1e-6, frob_max(a, b)
+    assert frob_max(a, U0) > 1e-3
+    c.close()
+
+
+@pytest.mark.parametrize("N,coupling", [(64, 3), (512, 4), (2048, 4)])
+def test_mixed_one_step_philox(jw, N, coupling):
+    """One step from the same U with the Philox stream: the FP32 Box-Muller
+    normals (same Philox words) and the fused FP32 passes against FP64."""
+    c = jw.Context(N)
+    U0 = mv_start(jw, c, Ny=2)
+    al = alpha_for(jw, N, coupling)
+    ds = 5e-5 if coupling == 4 else 1e-4
+    U64, U32 = c.field(9, U0), c.field(9, U0)
+    c.philox(17, 3, 40)
+    c.evolve(U64, coupling, 1, ds, al, nsteps=1)
+    c.philox(17, 3, 40)
     c.evolve_mixed(U32, coupling, 1, ds, al, nsteps=1, reunit_every=0)
     a, b = U64.download(), U32.download()
     assert frob_max(a, b) <= 1e-6, frob_max(a, b)
