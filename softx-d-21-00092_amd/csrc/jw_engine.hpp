@@ -175,7 +175,8 @@ __device__ __forceinline__ void row_noise_body(C* lds, C* __restrict__ Xi, size_
     T m[SPT][8];
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
-        const int y = tid + q * NT;
+        // FP32: sites one after another (no spills at two workgroups per CU)
+        const int y = sizeof(C) == sizeof(float2) ? site_of<C>(tid + q * NT) : tid + q * NT;
         T z[16];
         normals16_t(ns, row0 + y, z);
         const double s = sa_pos ? sa_pos[row0 + y] : 1.0;
