@@ -409,8 +409,8 @@ struct padded { static constexpr int value = N + (PS ? (N >> PS) : 0); };
 template <int REM>
 struct radix_of { static constexpr int value = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2); };
 
-template <int N, int P, int R, int NSEQ, int ES, int SS, int NT, int DIR, int PS, typename C>
-__device__ __forceinline__ void fft_stage(C* buf, const C* __restrict__ tw, int tid) {
+template <int N, int P, int R, int NSEQ, int ES, int SS, int NT, int DIR, int PS, typename C, typename TW>
+__device__ __forceinline__ void fft_stage(C* buf, TW tw, int tid) {
     constexpr int S = N / R;
     constexpr int NB = S * NSEQ;
     constexpr int PER = (NB + NT - 1) / NT;
@@ -460,8 +460,8 @@ __device__ __forceinline__ void fft_stage(C* buf, const C* __restrict__ tw, int 
 
 template <int N, int P, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
 struct fft_stages {
-    template <typename C>
-    static __device__ __forceinline__ void run(C* buf, const C* __restrict__ tw, int tid) {
+    template <typename C, typename TW>
+    static __device__ __forceinline__ void run(C* buf, TW tw, int tid) {
         constexpr int R = radix_of<N / P>::value;
         fft_stage<N, P, R, NSEQ, ES, SS, NT, DIR, PS>(buf, tw, tid);
         fft_stages<N, P * R, NSEQ, ES, SS, NT, DIR, PS>::run(buf, tw, tid);
@@ -469,14 +469,60 @@ struct fft_stages {
 };
 template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
 struct fft_stages<N, N, NSEQ, ES, SS, NT, DIR, PS> {
-    template <typename C>
-    static __device__ __forceinline__ void run(C*, const C* __restrict__, int) {}
+    template <typename C, typename TW>
+    static __device__ __forceinline__ void run(C*, TW, int) {}
 };
 
-// C = double2 (FP64 form) or float2 (mixed form), deduced from buf and tw
-template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS = 0, typename C>
-__device__ __forceinline__ void lds_fft(C* buf, const C* __restrict__ tw, int tid) {
+// C = double2 (FP64 form) or float2 (mixed form), deduced from buf; tw is the
+// global twiddle table (const C*) or an LDS two-level table (TwL)
+template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS = 0, typename C, typename TW>
+__device__ __forceinline__ void lds_fft(C* buf, TW tw, int tid) {
     fft_stages<N, 1, NSEQ, ES, SS, NT, DIR, PS>::run(buf, tw, tid);
+}
+
+// Twiddles w^m = exp(-2 pi i m / N) from two small LDS tables holding entries
+// of the global table: lo[i] = w^i (i < L), hi[j] = w^(j L) (j < N / L), and
+// w^m = hi[m / L] lo[m mod L] (one complex product, about 2 ulp).  A stage then
+// waits on LDS (tens of cycles) instead of a global load that the streamed
+// tiles keep evicting from L1.
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+template <int N>
+struct tw_split {
+    static constexpr int LB = (ilog2c(N) + 1) / 2, L = 1 << LB, H = N / L;
+};
+template <typename C, int N>
+struct TwL {
+    const C* t;     // lo (L entries), then hi (H entries)
+    __device__ __forceinline__ C operator[](int m) const {
+        return cmul(t[tw_split<N>::L + (m >> tw_split<N>::LB)], t[m & (tw_split<N>::L - 1)]);
+    }
+};
+// the twiddle source of a kernel: the LDS tables (USE_LDS) or the global table
+template <int N, bool USE_LDS, typename C>
+__device__ __forceinline__ auto tw_source(const C* __restrict__ tw);
+// as passed to one FFT: a global table pointer made opaque (its loads are not
+// kept live from one FFT to the next), LDS tables as they are
+template <typename C>
+__device__ __forceinline__ const C* tw_fft(const C* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+template <typename C, int N>
+__device__ __forceinline__ TwL<C, N> tw_fft(TwL<C, N> t) { return t; }
+
+// fills the workgroup's tables from the global table (one barrier)
+template <int N, typename C>
+__device__ __forceinline__ TwL<C, N> lds_twiddles(const C* __restrict__ tw) {
+    constexpr int L = tw_split<N>::L, H = tw_split<N>::H;
+    __shared__ C t[L + H];
+    for (int i = threadIdx.x; i < L + H; i += blockDim.x) t[i] = i < L ? tw[i] : tw[(i - L) * L];
+    __syncthreads();
+    return TwL<C, N>{t};
+}
+template <int N, bool USE_LDS, typename C>
+__device__ __forceinline__ auto tw_source(const C* __restrict__ tw) {
+    if constexpr (USE_LDS) return lds_twiddles<N>(tw);
+    else return tw;
 }
 
 }  // namespace jw

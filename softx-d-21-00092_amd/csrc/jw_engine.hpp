@@ -143,9 +143,9 @@ __device__ __forceinline__ int opaque_v(int v) {
 
 // inverse row FFT of 4 planes in LDS; in FP32 through opaque twiddles and
 // thread index (fewer live registers around the site algebra)
-template <int N, int NT, int PN, typename C>
-__device__ __forceinline__ void row_ifft4(C* lds, const C* __restrict__ tw, int tid) {
-    if constexpr (sizeof(C) == sizeof(float2) || N <= 1024) lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, opaque(tw), opaque_v(tid));
+template <int N, int NT, int PN, typename C, typename TW>
+__device__ __forceinline__ void row_ifft4(C* lds, TW tw, int tid) {
+    if constexpr (sizeof(C) == sizeof(float2) || N <= 1024) lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw_fft(tw), opaque_v(tid));
     else lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
 }
 
@@ -163,10 +163,9 @@ __device__ __forceinline__ int site_of(int y) {
 // for the next step after its own sites (the noise is compute-bound, the
 // update memory-bound).  lds: 4 padded rows.
 // ---------------------------------------------------------------------------
-template <int N, int BF, typename C>
+template <int N, int BF, typename C, typename TW>
 __device__ __forceinline__ void row_noise_body(C* lds, C* __restrict__ Xi, size_t V, const NoiseSrc& ns,
-                                               const double* __restrict__ sa_pos, const C* __restrict__ tw,
-                                               int x) {
+                                               const double* __restrict__ sa_pos, TW tw, int x) {
     typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
@@ -210,7 +209,8 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_noise_f
                                                               const C* __restrict__ tw) {
     constexpr int PN = padded<N, 4>::value;
     __shared__ C lds[4 * PN];
-    row_noise_body<N, BF, C>(lds, Xi + zofs(8, V), V, batch_noise(ns, V), sa_pos, tw, row_of_block<N, BF>(blockIdx.x));
+    row_noise_body<N, BF, C>(lds, Xi + zofs(8, V), V, batch_noise(ns, V), sa_pos,
+                             tw_source<N, true>(tw), row_of_block<N, BF>(blockIdx.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -226,6 +226,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_fw
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
     __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
+    const auto twl = tw_source<N, true>(tw);
     Vv += zofs(8, V);
     U += zofs(9, V);
     ns = batch_noise(ns, V);
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_fw
         for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mkc<C>(vx[2 * p], vx[2 * p + 1]);
     }
     __syncthreads();
-    lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
+    lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, twl, tid);
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_fw
 #pragma unroll
         for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = mkc<C>(vy[q][2 * p], vy[q][2 * p + 1]);
     __syncthreads();
-    lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
+    lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, twl, tid);
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -294,6 +295,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
     constexpr int PN = padded<N, 4>::value;
     __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
+    const auto twl = tw_source<N, false>(tw);
     Vv += zofs(8, V);
     U += zofs(9, V);
     Xs += zofs(8, V);
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
             for (int q = 0; q < SPT; q++)
                 lds[lpos<1, PN, 4>(tid + q * NT, p)] = Xs[pat<N, BF>(4 * half + p, x, tid + q * NT, V)];
         __syncthreads();
-        lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
+        lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, twl, tid);
         // one site at a time: keeps a single U (9 complex) live
 #pragma unroll 1
         for (int q = 0; q < SPT; q++) {
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
             for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mkc<C>(v[2 * p], v[2 * p + 1]);
         }
         __syncthreads();
-        lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
+        lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, twl, tid);
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -362,6 +364,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
     constexpr int S2 = 2 * W;
     __shared__ C lds[padded<N, 3>::value * S2];
     const int tid = threadIdx.x;
+    const auto twl = tw_source<N, sizeof(C) == sizeof(float2)>(tw);
     const int strip = xcd_strip(blockIdx.x, N / W);
     const int col0 = strip * W;
     Z += zofs(4, V);
@@ -390,7 +393,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
         lds[lpos<S2, 1, 3>(x, s)] = (PI || s < W) ? X[g] : Y[g];
     }
     __syncthreads();
-    lds_fft<N, S2, S2, 1, NT, -1, 3>(lds, opaque(tw), opaque_v(tid));
+    lds_fft<N, S2, S2, 1, NT, -1, 3>(lds, tw_fft(twl), opaque_v(tid));
     // pair (x, c): X^ in slot c of LDS row x, Y^ in slot W + c
     C zreg[CORR ? PAIRS : 1];
     // FP32 correlated: indices recomputed per phase (opaque_v), since addresses
@@ -425,7 +428,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
     __syncthreads();
     if (CORR) {
         // xi~ = F_col^-1 of the scaled tiles, written back over X and Y
-        lds_fft<N, S2, S2, 1, NT, +1, 3>(lds, opaque(tw), opaque_v(tid));
+        lds_fft<N, S2, S2, 1, NT, +1, 3>(lds, tw_fft(twl), opaque_v(tid));
         const int ts = REIDX ? opaque_v(tid) : tid;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
@@ -445,7 +448,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
         __syncthreads();
     }
     // Z: the W sequences in the X slots
-    lds_fft<N, W, S2, 1, NT, +1, 3>(lds, opaque(tw), opaque_v(tid));
+    lds_fft<N, W, S2, 1, NT, +1, 3>(lds, tw_fft(twl), opaque_v(tid));
     const int to = REIDX ? opaque_v(tid) : tid;
 #pragma unroll
     for (int q = 0; q < PAIRS; q++) {
@@ -493,6 +496,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __res
     constexpr int PN = padded<N, 4>::value;
     __shared__ C lds[CPB * PN];
     const int tid = threadIdx.x;
+    const auto twl = tw_source<N, true>(tw);
     Z += zofs(4, V);
     X += zofs(8, V);
     Y += zofs(8, V);
@@ -527,13 +531,12 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __res
     // with f_d = -2 pi cc sb[k_d] / phat^2 / V, zero at k = 0; CORR: scale P^
     // by sa and write F_col^-1[sa P^ / V] back over P
     auto direction = [&](C* __restrict__ P, C acc[PER], int d) {
-        const C* twd = opaque(tw);
         const double* q4d = opaque(q4);
         const double* sbd = opaque(sb);
         const double* sad = opaque(sa16);
         load(P);
         __syncthreads();
-        lds_fft<N, CPB, 1, PN, NT, -1, 4>(lds, twd, opaque_v(tid));
+        lds_fft<N, CPB, 1, PN, NT, -1, 4>(lds, tw_fft(twl), opaque_v(tid));
         const int t = opaque_v(tid);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
@@ -557,7 +560,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __res
         }
         __syncthreads();
         if (CORR) {
-            lds_fft<N, CPB, 1, PN, NT, +1, 4>(lds, opaque(tw), opaque_v(tid));
+            lds_fft<N, CPB, 1, PN, NT, +1, 4>(lds, tw_fft(twl), opaque_v(tid));
             store(P);
             __syncthreads();
         }
@@ -575,7 +578,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __res
         lds[lpos<1, PN, 4>(pr % N, pr / N)] = acc[q];
     }
     __syncthreads();
-    lds_fft<N, CPB, 1, PN, NT, +1, 4>(lds, opaque(tw), opaque_v(tid));
+    lds_fft<N, CPB, 1, PN, NT, +1, 4>(lds, tw_fft(twl), opaque_v(tid));
     store(Z + po);
 }
 
@@ -599,6 +602,7 @@ __global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_up
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
     __shared__ C lds[4 * PN];
     const int tid = threadIdx.x;
+    const auto twl = tw_source<N, sizeof(C) == sizeof(float2)>(tw);
     U += zofs(9, V);
     ZA += zofs(4, V);
     ZB += zofs(4, V);
@@ -609,7 +613,7 @@ __global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_up
 #pragma unroll
         for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZA[p * V + lay<N, BF>(x, tid + q * NT)];
     __syncthreads();
-    row_ifft4<N, NT, PN>(lds, tw, tid);
+    row_ifft4<N, NT, PN>(lds, twl, tid);
     T ha[SPT][8];
 #pragma unroll
     for (int q = 0; q < SPT; q++)
@@ -625,7 +629,7 @@ __global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_up
 #pragma unroll
         for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tid + q * NT, p)] = ZB[p * V + lay<N, BF>(x, tid + q * NT)];
     __syncthreads();
-    row_ifft4<N, NT, PN>(lds, tw, tid);
+    row_ifft4<N, NT, PN>(lds, twl, tid);
 #pragma unroll
     for (int q = 0; q < SPT; q++) {
         const int y = site_of<C>(tid + q * NT);
@@ -680,7 +684,7 @@ __global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_up
         __syncthreads();
         NoiseSrc nn = batch_noise(ns, V);
         nn.step += 1;
-        row_noise_body<N, BF, C>(lds, Xi + zofs(8, V), V, nn, sa_pos, opaque(tw), x);
+        row_noise_body<N, BF, C>(lds, Xi + zofs(8, V), V, nn, sa_pos, tw_fft(twl), x);
     }
 }
 
