@@ -11,7 +11,7 @@ Synthetic data: all noise comes from the Philox stream.
 
 Forms, each timed over the same K steps from the same start:
   * fp64  -- jw_evolve: FP64 throughout (the oracle-parity path);
-  * mixed -- jw_evolve_mixed: FP32 fields and FFTs, FP64 exponentials, U
+  * mixed -- jw_evolve_mixed: FP32 fields, FFTs and products, FP64 exponentials, U
              projected onto SU(3) every 10 steps;
   * mixed+batched -- jw_batch_evolve with --batch N = 2048 trajectories in one
              batch (FP32, captured step graph, tuned layout).
@@ -298,9 +298,9 @@ def main():
     head = "mixed" if "mixed" in res else forms[0]
     h = res[head]
     value = batched["traj_steps_per_s"] if batched else h["steps_per_s"]
-    form_txt = (f"mixed+batched: {a.batch} trajectories of N={N} in one batch, FP32 fields and FFTs, FP64 "
+    form_txt = (f"mixed+batched: {a.batch} trajectories of N={N} in one batch, FP32 fields, FFTs and products, FP64 "
                 f"exponentials, SU(3) projection every {a.reunit_every} steps" if batched else
-                ("mixed: FP32 fields and FFTs, FP64 exponentials, SU(3) projection every "
+                ("mixed: FP32 fields, FFTs and products, FP64 exponentials, SU(3) projection every "
                  f"{a.reunit_every} steps; one trajectory") if head == "mixed" else "plain FP64")
     out = {"metric": METRIC, "value": value, "unit": "trajectory-steps/s" if batched else "steps/s",
            "n_gpus": 1, "steps": a.steps,
