@@ -149,9 +149,10 @@ __device__ __forceinline__ void row_ifft4(C* lds, TW tw, int tid) {
     else lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, tw, tid);
 }
 
-// a thread's site index in a row pass's site loop; in FP32 opaque, so that the
-// compiler runs the sites one after another instead of interleaving their
-// site algebra (registers for two workgroups per CU)
+// a thread's site index in a row pass's site loop, opaque, so that the compiler
+// runs the sites one after another instead of interleaving their site algebra
+// (FP32: registers for two workgroups per CU; FP64: fewer spills, the same
+// arithmetic)
 template <typename C>
 __device__ __forceinline__ int site_of(int y) {
     return opaque_v(y);
