@@ -291,7 +291,7 @@ __device__ __forceinline__ void box_muller_pair(uint32_t site, uint32_t step, ui
 }
 
 // The same pair in FP32 from the same Philox words (mixed-precision form):
-// log u0 from the 53-bit integer (through log1p of 1 - u0 above 1/2, so that
+// log u0 from the 53-bit integer (as log1p of -(1 - u0) above 1/2, so that
 // small |z| keep their relative precision) and the angle as pi x with
 // x = 2 u1 - 1 in (-1, 1): sincospi(x + 1) = -sincospi(x).  Each normal agrees
 // with the FP64 pair to a few FP32 ulps of max(|z|, 1).
@@ -301,13 +301,20 @@ __device__ __forceinline__ void box_muller_pair_f(uint32_t site, uint32_t step, 
     philox4x32_10(c, seed, traj);
     const uint64_t m0 = (((uint64_t)c[0] << 32) | c[1]) >> 11;     // u0 = (m0 + 1/2) 2^-53
     const uint64_t m1 = (((uint64_t)c[2] << 32) | c[3]) >> 11;     // u1 = (m1 + 1/2) 2^-53
-    float lu;
-    if (m0 < (1ull << 52)) lu = logf(((float)m0 + 0.5f) * 0x1.0p-53f);
-    else lu = log1pf(-((float)((1ull << 53) - 1 - m0) + 0.5f) * 0x1.0p-53f);
+    // 53-bit integers to FP32 through FP64 (exact there, then one rounding: the
+    // same value as a direct conversion, in fewer instructions)
+    // One logarithm for both halves: u0 < 1/2 takes log u0; above, with
+    // t = 1 - u0 from the integer and a = 1 - t rounded (1 - a exact),
+    // log1p(-t) = log(a) t / (1 - a) (Goldberg's log1p), or -t when a = 1.
+    const bool lo = m0 < (1ull << 52);
+    const float t = ((float)(double)((1ull << 53) - 1 - m0) + 0.5f) * 0x1.0p-53f;
+    const float a = lo ? ((float)(double)m0 + 0.5f) * 0x1.0p-53f : 1.0f - t;
+    float lu = logf(a);
+    if (!lo) lu = a == 1.0f ? -t : lu * (t * __builtin_amdgcn_rcpf(1.0f - a));
     const float rr = sqrtf(-2.0f * lu);
     const int64_t d = 2 * (int64_t)m1 + 1 - (1ll << 53);            // x = d 2^-53
     float s, co;
-    sincospif((float)d * 0x1.0p-53f, &s, &co);
+    sincospif((float)(double)d * 0x1.0p-53f, &s, &co);
     z0 = -rr * co;
     z1 = -rr * s;
 }

@@ -179,11 +179,11 @@ __device__ __forceinline__ void row_noise_body(C* lds, C* __restrict__ Xi, size_
         const int y = sizeof(C) == sizeof(float2) ? site_of<C>(tid + q * NT) : tid + q * NT;
         T z[16];
         normals16_t(ns, row0 + y, z);
-        const double s = sa_pos ? sa_pos[row0 + y] : 1.0;
+        const T s = sa_pos ? (T)sa_pos[row0 + y] : (T)1;      // in the element type: no FP64 in FP32
 #pragma unroll
-        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = mkc<C>(z[4 * p] * s, z[4 * p + 2] * s);
+        for (int p = 0; p < 4; p++) lds[lpos<1, PN, 4>(y, p)] = C{z[4 * p] * s, z[4 * p + 2] * s};
 #pragma unroll
-        for (int k = 0; k < 8; k++) m[q][k] = (T)(z[2 * k + 1] * s);
+        for (int k = 0; k < 8; k++) m[q][k] = z[2 * k + 1] * s;
     }
     __syncthreads();
     lds_fft<N, 4, 1, PN, NT, -1, 4>(lds, tw, tid);
