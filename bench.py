@@ -128,8 +128,8 @@ def kernel_table(kt, nk, V, planes, bytes_per_real):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              f"(the lattice would shard as x-slabs with an all-to-all per column pass, see DESIGN.md)",
+        print(f"bench.py: this solver runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(the lattice would shard as x-slabs with an all-to-all per column pass)",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np

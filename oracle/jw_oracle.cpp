@@ -16,8 +16,8 @@
 // field.h:708-710 for the MV charges: n0..n7).  The Philox stream used by the HIP
 // library is restated here too (Salmon et al., SC'11), so the two can be compared.
 //
-// Parity pins (which check fixes which function) are listed in DESIGN.md's oracle
-// table; each function below names the reference lines it restates.
+// Each function below names the reference lines it restates; the tests that pin
+// it are in tests/test_oracle.py.
 
 #include <cmath>
 #include <complex>

@@ -5,7 +5,7 @@
 //
 // TEST INFRASTRUCTURE ONLY: used by tests/ to pin the CPU oracle against the
 // reference itself.  FFTW (needed by mpi_fftw_class.h and the drivers) is not in
-// this image, so the transforms are not part of this build.
+// this image, so the transforms are not compiled here.
 //
 // All field arguments are 64 x 64 AoS arrays of complex<double> (field.h:199-209).
 

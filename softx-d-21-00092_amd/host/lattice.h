@@ -30,7 +30,7 @@ class mpi_class {
     mpi_class() {}
     int mpi_init(config* cnfg) {
         if (proc_grid[0] != 1 || proc_grid[1] != 1) {
-            fprintf(stderr, "mpi_class: process grid %d x %d requested; this build runs one lattice per GPU (1 x 1)\n",
+            fprintf(stderr, "mpi_class: process grid %d x %d requested; this solver runs one lattice per GPU (1 x 1)\n",
                     proc_grid[0], proc_grid[1]);
             exit(2);
         }
