@@ -138,13 +138,16 @@ __device__ __forceinline__ void gen_from_upper(const m3& M, double h[8]) {
 // With X^3 = c1 X + c0 (c1 = tr(X^2)/2, c0 = det X), X^n = a_n + b_n X + d_n X^2
 // where (a, b, d)_{n+1} = (c0 d_n, a_n + c1 d_n, b_n); hence
 //   exp(iX) = f0 + f1 X + f2 X^2,  f = sum_n i^n (a_n, b_n, d_n) / n!.
-// The sum runs to 24 terms for spectral radius rho <= 2 sqrt(c1/3) <= 1.6
-// (c1 <= 1.92): the remainder is below rho^25/25! ~ 1e-20.  Larger X are
-// scaled by 2^-k (exact) into that range and the result squared k times.
+// The sum runs to at most 24 terms for spectral radius rho <= 2 sqrt(c1/3)
+// <= 1.6 (c1 <= 1.92): the remainder is below rho^25/25! ~ 1e-20; it stops
+// earlier once a term's |a| + |b| + |d| is below 1e-17 (small X: a Langevin
+// step's exponent converges in about 10 terms).  Larger X are scaled by 2^-k
+// (exact) into that range and the result squared k times.
 // X = 0 gives exactly 1.
 // ---------------------------------------------------------------------------
 #define JW_EXP_SERIES_C1 1.92
 #define JW_EXP_SERIES_TERMS 24
+#define JW_EXP_SERIES_TAIL 1e-17
 
 __device__ __forceinline__ m3 exp_i_gen(const double h[8], double s) {
     double c1 = 0.0;
@@ -182,6 +185,9 @@ __device__ __forceinline__ m3 exp_i_gen(const double h[8], double s) {
         else if ((n & 3) == 1) { f0i += A; f1i += B; f2i += D; }
         else if ((n & 3) == 2) { f0r -= A; f1r -= B; f2r -= D; }
         else { f0i -= A; f1i -= B; f2i -= D; }
+        // the rest of the series is below 4e-17 in E (terms fall at least 5x per
+        // step from here, |X| <= 1.6): stop (Langevin steps converge in ~10 terms)
+        if (fabs(A) + fabs(B) + fabs(D) < JW_EXP_SERIES_TAIL) break;
     }
     const cplx f0 = mk(f0r, f0i), f1 = mk(f1r, f1i), f2 = mk(f2r, f2i);
     const m3 X2 = mmul(X, X);
@@ -233,6 +239,8 @@ __device__ __forceinline__ m3f exp_i_gen_f(const H h[8], double s) {
         else if ((n & 3) == 1) { f0i += A; f1i += B; f2i += D; }
         else if ((n & 3) == 2) { f0r -= A; f1r -= B; f2r -= D; }
         else { f0i -= A; f1i -= B; f2i -= D; }
+        // the result is rounded to FP32: a tail below 1e-12 cannot show
+        if (fabs(A) + fabs(B) + fabs(D) < 1e-13) break;
     }
     const cplx f1 = mk(f1r, f1i), f2 = mk(f2r, f2i);
     // X^2: real diagonal and upper triangle
