@@ -185,6 +185,18 @@ class Oracle:
         self._ok(self.L.or_reduce(N, _p(c), _p(s), _p(e)))
         return s, e
 
+    def scale_by_sqrt_alpha(self, f, N, al=None):
+        f = np.ascontiguousarray(f, dtype=np.complex128).copy()
+        t = f.size // (N * N)
+        self._ok(self.L.or_scale_by_sqrt_alpha(N, t, _p(f), ctypes.byref(al) if al else None))
+        return f
+
+    def correlate_noise(self, f, N, al=None):
+        f = np.ascontiguousarray(f, dtype=np.complex128).copy()
+        t = f.size // (N * N)
+        self._ok(self.L.or_correlate_noise(N, t, _p(f), ctypes.byref(al) if al else None))
+        return f
+
     def langevin_step(self, U, z, N, coupling, kernel, step, al=None):
         U = np.ascontiguousarray(U, dtype=np.complex128).copy()
         z = np.ascontiguousarray(z, dtype=np.float64)
@@ -318,3 +330,9 @@ class Reference:
         c = np.ascontiguousarray(c, dtype=np.complex128)
         self.L.ref_reduce(_p(c), _p(s), _p(e))
         return s, e
+
+    def correlations(self):
+        """setCorrelationsForCouplingConstant(momenta*): alpha(phat2(k)) / sqrt(V)."""
+        c = _field(self.N, 1)
+        self.L.ref_correlations(_p(c))
+        return c

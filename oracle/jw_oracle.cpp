@@ -674,11 +674,41 @@ int or_reduce(int N, const double* corr, double* sum, double* err) {
 }
 
 // ---- composed driver pieces -------------------------------------------------
+// Per-mode correlated noise (coupling 4), in momentum space: each of the t
+// components of the spectrum f is multiplied by sqrt(alpha(phat2(k))), alpha
+// in prepare_A_local's operation order (field.h:2045-2046).  With xi~ =
+// F^-1[sqrt(alpha) F xi] the noise covariance is the circulant
+// C(x - y) = (1/V) sum_k alpha(k) e^{ik(x-y)}: the matrix whose dense Cholesky
+// factor the reference's POSITION_EVOLUTION path applies (field.h:1633-1651
+// builds alpha(k)/sqrt(V), main_optimized.cpp:145-157 transforms it backward
+// and factors it with matrix.h:151-200).  The per-mode factor sqrt(alpha) is the
+// Fourier-diagonal square root of that matrix.
+int or_scale_by_sqrt_alpha(int N, int t, double* f, const or_alpha* ap) {
+    if (!pow2(N) || t < 1) return -1;
+    const or_alpha a = ap ? *ap : alpha_default();
+    const size_t V = (size_t)N * N;
+    std::vector<double> ph, bx, by;
+    momenta_tables(N, ph, bx, by);
+    for (size_t i = 0; i < V; i++) {
+        const double s = std::sqrt(alpha_kernel(ph[i], N, a));
+        for (int k = 0; k < 2 * t; k++) f[2 * t * i + k] *= s;
+    }
+    return 0;
+}
+
+// xi~ = F^-1[sqrt(alpha(k)) F xi] on t-component position-space fields, in place
+// (the map of or_langevin_step's coupling 4, exposed for the covariance pin).
+int or_correlate_noise(int N, int t, double* f, const or_alpha* ap) {
+    if (or_fft2d(N, t, f, 1)) return -1;
+    if (or_scale_by_sqrt_alpha(N, t, f, ap)) return -1;
+    return or_fft2d(N, t, f, 0);
+}
+
 // One Langevin step of main_optimized's MOMENTUM branch (main_optimized.cpp:243-304)
 // with injected normals z (16 per site).  coupling as in or_prepare_A_local;
 // NOISE uses the position-index noise scaling (field.h:2273);
-// CORRELATED multiplies F xi by sqrt(alpha(phat2(k))) before both the kernel and
-// the inverse transform (per-mode factor of the coupling covariance).
+// CORRELATED multiplies F xi by sqrt(alpha(phat2(k))) (or_scale_by_sqrt_alpha)
+// before both the kernel (factor 1) and the inverse transform.
 int or_langevin_step(int N, double* U, const double* z, int coupling, int kernel, double step,
                      const or_alpha* ap) {
     if (!pow2(N) || coupling == 2 || coupling < 0 || coupling > 4) return -1;
@@ -690,12 +720,8 @@ int or_langevin_step(int N, double* U, const double* z, int coupling, int kernel
     or_fft2d(N, 9, xix.data(), 1);
     or_fft2d(N, 9, xiy.data(), 1);
     if (coupling == 4) {
-        std::vector<double> ph, bx, by;
-        momenta_tables(N, ph, bx, by);
-        for (size_t i = 0; i < V; i++) {
-            const double s = std::sqrt(alpha_kernel(ph[i], N, a));
-            for (int k = 0; k < 18; k++) { xix[18 * i + k] *= s; xiy[18 * i + k] *= s; }
-        }
+        or_scale_by_sqrt_alpha(N, 9, xix.data(), &a);
+        or_scale_by_sqrt_alpha(N, 9, xiy.data(), &a);
     }
     or_prepare_A_local(N, A.data(), xix.data(), xiy.data(), coupling, kernel, &a);
     or_fft2d(N, 9, A.data(), 0);

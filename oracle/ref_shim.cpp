@@ -180,6 +180,16 @@ int ref_average_and_symmetrize(double* corr) {
     return 0;
 }
 
+// lfield<double,1>::setCorrelationsForCouplingConstant(momenta*), field.h:1633-1651:
+// alpha(phat2(k)) / sqrt(V) per mode, the input main_optimized.cpp:145-157
+// transforms backward and factors densely (matrix.h:151-200)
+int ref_correlations(double* corr) {
+    lfield<double, 1> c(Nx, Ny);
+    c.setCorrelationsForCouplingConstant(single_momenta());
+    store(c, corr);
+    return 0;
+}
+
 // gfield::reduce(sum, err, mpi), field.h:1521-1536
 int ref_reduce(const double* corr, double* sum, double* err) {
     gfield<double, 1> g(Nx, Ny);

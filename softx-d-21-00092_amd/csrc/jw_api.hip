@@ -235,12 +235,15 @@ hipError_t batch_group_step(jw_batch* b, BatchGroup& g, int coupling, double ste
 
 // one step of every group, forked from and joined back to b->stream, then the
 // step counter; capturable.  first/next: as StepArgs::first_noise/next_noise.
+// Each group stream waits on the fork before anything is enqueued on it, the
+// coupling-table check included (the tables are built before a capture, so
+// inside one nothing is rebuilt).
 int batch_step_all(jw_batch* b, int coupling, double step, const jw::Alpha& al, int reunit, int first, int next) {
     HIP_CHECK(hipEventRecord(b->fork, b->stream));
     for (auto& g : b->g) {
         const double *sa_pos, *sa_k;
-        if (int e = batch_tables(g, coupling, al, &sa_pos, &sa_k)) return e;
         HIP_CHECK(hipStreamWaitEvent(g.ctx->stream, b->fork, 0));
+        if (int e = batch_tables(g, coupling, al, &sa_pos, &sa_k)) return e;
         HIP_CHECK(batch_group_step(b, g, coupling, step, sa_pos, sa_k, reunit, g.count, g.U, first, next));
         HIP_CHECK(hipEventRecord(g.join, g.ctx->stream));
         HIP_CHECK(hipStreamWaitEvent(b->stream, g.join, 0));
@@ -251,37 +254,44 @@ int batch_step_all(jw_batch* b, int coupling, double step, const jw::Alpha& al, 
 
 // The tuner: for each group with no layout yet, time 3 steps (after one
 // untimed) of each intermediate layout on a copy of its U with a scratch noise
-// key, and keep the fastest.  The trajectories and the step counter are left
-// as they were.
+// key, and keep the fastest.  A layout whose tables or steps fail is skipped;
+// the tuning fails only when no layout ran.  The trajectories and the step
+// counter are left as they were.
 int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
-    for (auto& g : b->g) {
+    for (size_t gi = 0; gi < b->g.size(); gi++) {
+        BatchGroup& g = b->g[gi];
         if (g.layout >= 0) continue;
         const size_t ubytes = (size_t)g.count * 9 * g.ctx->V * elem_size(b);
         void* scratch = nullptr;
         if (hipMalloc(&scratch, ubytes) != hipSuccess) return fail(JW_ERR_ALLOC, "tuning buffer allocation failed");
-        hipMemcpyAsync(scratch, g.U, ubytes, hipMemcpyDeviceToDevice, g.ctx->stream);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipMemcpyAsync(scratch, g.U, ubytes, hipMemcpyDeviceToDevice, g.ctx->stream) != hipSuccess ||
+            hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            if (e0) hipEventDestroy(e0);
+            hipStreamSynchronize(g.ctx->stream);
+            hipFree(scratch);
+            return fail(JW_ERR_HIP, "tuning set-up failed for group %zu", gi);
+        }
         const uint32_t saved_traj0 = b->traj0;
         b->traj0 = 0xFFFFF000u;      // scratch noise key
-        hipEvent_t e0, e1;
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
         float best = 1e30f;
-        int best_layout = 0;
-        int rc = JW_OK;
+        int best_layout = -1;
         for (int lay : {0, 2, 4, 8}) {
             g.ctx->layout = lay;
             const double *sa_pos, *sa_k;
-            if ((rc = batch_tables(g, coupling, al, &sa_pos, &sa_k))) break;
+            if (batch_tables(g, coupling, al, &sa_pos, &sa_k) != JW_OK) continue;
+            bool ok = batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, scratch, 1, 0) == hipSuccess;
+            ok = ok && hipEventRecord(e0, g.ctx->stream) == hipSuccess;
+            for (int it = 0; it < 3 && ok; it++)
+                ok = batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, scratch, 1, 0) == hipSuccess;
+            ok = ok && hipEventRecord(e1, g.ctx->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
             float ms = 0.f;
-            for (int it = 0; it < 4 && rc == JW_OK; it++) {
-                if (it == 1) hipEventRecord(e0, g.ctx->stream);
-                if (batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, scratch, 1, 0) != hipSuccess)
-                    rc = fail(JW_ERR_HIP, "tuning step failed");
+            ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            if (!ok) {
+                (void)hipGetLastError();
+                continue;
             }
-            hipEventRecord(e1, g.ctx->stream);
-            hipEventSynchronize(e1);
-            hipEventElapsedTime(&ms, e0, e1);
-            if (rc == JW_OK && ms < best) {
+            if (ms < best) {
                 best = ms;
                 best_layout = lay;
             }
@@ -291,7 +301,10 @@ int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
         b->traj0 = saved_traj0;
         hipStreamSynchronize(g.ctx->stream);
         hipFree(scratch);
-        if (rc) return rc;
+        if (best_layout < 0) {
+            g.ctx->layout = -1;
+            return fail(JW_ERR_HIP, "tuning: no intermediate layout ran for group %zu", gi);
+        }
         g.layout = best_layout;
         g.ctx->layout = best_layout;
     }
@@ -1019,14 +1032,22 @@ int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw
     if (!b->fp32) reunit_every = 0;      // the FP64 form is not projected (as jw_evolve)
     HIP_CHECK(hipSetDevice(b->device));
     const jw::Alpha al = to_alpha(a);
-    if (int e = batch_tune(b, coupling, step, al)) return e;
     // steps per graph: one projection period (10 without projection)
     const int K = reunit_every > 0 ? reunit_every : 10;
     const BatchKey key{coupling, reunit_every, step, al};
-    if (b->graph && !(b->have_key && key == b->key)) {
+    const bool same_key = b->have_key && key == b->key;
+    // A new coupling, step or alpha may rebuild the coupling tables that the
+    // previous call's graph (still running on b->stream) reads, and replaces
+    // that graph: wait for the previous call's work first.
+    if (!same_key) HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (int e = batch_tune(b, coupling, step, al)) return e;
+    if (b->graph && !same_key) {
         hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
+    // from here on the tables (and the graph, once built) are this key's
+    b->key = key;
+    b->have_key = true;
     // every group's stream must be idle before the origin stream forks to it
     for (auto& g : b->g) HIP_CHECK(hipStreamSynchronize(g.ctx->stream));
     if (!b->graph && nsteps >= K) {
@@ -1054,8 +1075,6 @@ int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw
             return fail(JW_ERR_HIP, "step graph: %s", hipGetErrorString(ei));
         }
         b->graph_steps = K;
-        b->key = key;
-        b->have_key = true;
     }
     int done = 0;
     if (b->graph)

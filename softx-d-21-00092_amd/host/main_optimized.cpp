@@ -77,9 +77,13 @@ int main(int argc, char* argv[]) {
         }
         const int every = cnfg->langevin_steps / cnfg->measurements > 0 ? cnfg->langevin_steps / cnfg->measurements : 1;
 
-        // the reference measures after Langevin step l when l % every == 0 and
-        // l / every < measurements (main_optimized.cpp:317-319)
-        auto measured = [&](int l) { return l % every == 0 && l / every < cnfg->measurements; };
+        // the reference measures after Langevin step l when l % every == 0, into
+        // slot l * measurements / langevin_steps (integer division), several
+        // measurements accumulating in one slot when measurements does not
+        // divide langevin_steps (main_optimized.cpp:317-336).  every = 1 where
+        // the reference would divide by zero (measurements > langevin_steps).
+        auto measured = [&](int l) { return l % every == 0; };
+        auto slot = [&](int l) { return (int)((long long)l * cnfg->measurements / cnfg->langevin_steps); };
         // steps from l up to and including the next measured one (or the end)
         auto chunk = [&](int l) {
             int n = 1;
@@ -140,7 +144,7 @@ int main(int argc, char* argv[]) {
                 if (measured(langevin - 1))
                     for (int stat = 0; stat < cnfg->stat; stat++) {
                         jwhost::check(jw_batch_get(batch, stat, uftmp.f), "batch get");
-                        measure(&uftmp, (langevin - 1) / every);
+                        measure(&uftmp, slot(langevin - 1));
                     }
             }
             jw_batch_destroy(batch);
@@ -183,7 +187,7 @@ int main(int argc, char* argv[]) {
                     jw_sync(jwhost::ctx());
                     std::cout << "Evolution time: " << seconds_since(starte) << std::endl;
                     langevin += n;
-                    if (measured(langevin - 1)) measure(&uftmp, (langevin - 1) / every);
+                    if (measured(langevin - 1)) measure(&uftmp, slot(langevin - 1));
                 }
                 std::cout << "Statistics time: " << seconds_since(start) << std::endl;
             }

@@ -62,6 +62,7 @@ def lib():
         L.jw_field_destroy.argtypes = [ctypes.c_void_p]
         L.jw_field_upload.argtypes = [ctypes.c_void_p, _dp]
         L.jw_field_download.argtypes = [ctypes.c_void_p, _dp]
+        L.jw_field_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.jw_noise_inject.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
         L.jw_noise_philox.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.jw_evolve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -166,6 +167,10 @@ class Context:
 
     def sync(self):
         _check(lib().jw_sync(self.h))
+
+    def copy(self, dst, src):
+        """lfield::operator= (jw_field_copy)."""
+        _check(lib().jw_field_copy(dst.h, src.h))
 
     # noise
     def inject(self, z, per_site=16):

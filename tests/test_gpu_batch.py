@@ -102,6 +102,44 @@ def test_batch_graph_replays(jw):
         c.close()
 
 
+@pytest.mark.parametrize("fp32", [1, 0])
+def test_batch_back_to_back_alpha_change(jw, fp32):
+    """Two evolutions with different running-coupling parameters issued back to back
+    (no sync in between: the second call rebuilds the coupling tables the first
+    call's graph reads) give the same trajectories as the same sequence with a sync
+    in between."""
+    import math
+    groups = [(128, 2), (64, 1)]
+    ctxs, states = mv_states(jw, groups)
+    al1 = jw.alpha(lambda_L=0.02 * 128, mu0_over_lambda=math.exp(2 * math.pi / (9 * 0.7)))
+    al2 = jw.alpha(lambda_L=0.05 * 128, mu0_over_lambda=math.exp(2 * math.pi / (9 * 0.5)))
+    out = []
+    for synced in (False, True):
+        b = jw.Batch(groups, fp32=fp32)
+        for g in range(len(groups)):
+            b.set_layout(g, 0)
+        for t, (N, U0) in enumerate(states):
+            b.set(t, ctxs[N].field(9, U0))
+        b.philox(5, 0, 0)
+        b.evolve(4, 1, 5e-5, al1, nsteps=20, reunit_every=10)
+        if synced:
+            b.sync()
+        b.evolve(4, 1, 5e-5, al2, nsteps=20, reunit_every=10)
+        b.sync()
+        res = []
+        for t, (N, _) in enumerate(states):
+            got = ctxs[N].field()
+            b.get(t, got)
+            res.append(got.download())
+        b.close()
+        out.append(res)
+    for t in range(len(states)):
+        assert np.array_equal(out[0][t], out[1][t]), t
+        assert frob_max(out[0][t], states[t][1]) > 1e-3
+    for c in ctxs.values():
+        c.close()
+
+
 def test_batch_tuned_layout(jw):
     groups = [(256, 2), (512, 1), (64, 4)]
     ctxs, states = mv_states(jw, groups, Ny=2)

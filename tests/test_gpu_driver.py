@@ -54,10 +54,11 @@ def oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, coupling, seed=
         for l in range(steps):
             zl = orc.philox_normals(N, seed, s, Ny + l, jw.TAG_LANGEVIN, 16)
             U = orc.langevin_step(U, zl, N, coupling, 1, ds)
-            if l % every == 0 and l // every < meas:
+            if l % every == 0:             # main_optimized.cpp:317-336
+                t = l * meas // steps
                 c = orc.measure_corr(U, N)[:, 0].real
-                sums[l // every] += c
-                errs[l // every] += c * c
+                sums[t] += c
+                errs[t] += c * c
     ph, _, _ = orc.momenta(N)
     x = 1.0 / 3.0 / stat
     rows = []
@@ -106,6 +107,22 @@ def test_driver_batched_and_mixed(jw, orc, tmp_path, engine, precision, rtol):
     assert got.shape == ref.shape and np.array_equal(got[:, :3], ref[:, :3])
     np.testing.assert_allclose(got[:, 3], ref[:, 3], rtol=0, atol=5e-7)     # printed with %f
     np.testing.assert_allclose(got[:, 4:], ref[:, 4:], rtol=rtol, atol=1e-12)
+
+
+def test_driver_measurement_schedule_uneven(jw, orc, tmp_path):
+    """langevin_steps = 10, measurements = 3: the reference measures after steps 0, 3, 6
+    and 9 into slots 0, 0, 1, 2 and prints slot 2, the state after step 9
+    (main_optimized.cpp:317-336, 353)."""
+    assert os.path.exists(DRIVER), "driver not built"
+    N, stat, mass, Ny, steps, meas, step = 32, 1, 2.0, 4, 10, 3, 0.0004
+    cfg = tmp_path / "cfg"
+    write_cfg(cfg, N, stat, mass, Ny, steps, meas, step, "FIXED_COUPLING", "fused")
+    r = subprocess.run([DRIVER, "1", "1", str(cfg)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = read_out(tmp_path / f"out_fused_fp64_{N}_{N}_mpi1_r0.dat")
+    ref = oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, 3)
+    assert got.shape == ref.shape and np.array_equal(got[:, :3], ref[:, :3])
+    np.testing.assert_allclose(got[:, 3:], ref[:, 3:], rtol=2e-6, atol=1e-12)
 
 
 def test_driver_rejects_multi_rank_grid(tmp_path):

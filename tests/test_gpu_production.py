@@ -1,0 +1,210 @@
+"""GPU parity at the production sizes of the benchmark configurations.
+
+  * N = 2048, per-mode correlated noise (alpha frozen at 0.7, Lambda a = 0.02,
+    Delta s = 5e-5): one FP64 step from the same uploaded U with identical
+    injected noise against the oracle's step (main_optimized.cpp:243-304),
+    per-site Frobenius <= 1e-12.
+  * N = 1024, MV start (Ny = 50, g^2 mu a = 0.25, m a = 1e-4), fixed coupling,
+    Delta s = 1e-4: two injected FP64 steps against the oracle, <= 1e-12.
+  * N = 2048, correlated: the mixed form against FP64 after 1000 steps of the
+    same Philox stream from the same MV start, S(r) within 2e-5 absolute.
+  * The uneven ensemble 48 x 256^2 + 32 x 512^2 + 16 x 1024^2 (fixed coupling,
+    Delta s = 1e-4): sampled trajectories against the same trajectory evolved
+    alone, and the ensemble-averaged S(r) of the mixed form against FP64 after
+    500 steps.
+  * N = 8192 (MV start Ny = 100, g^2 mu a = 0.05; correlated noise): a few steps
+    of both forms, S(r) of the mixed form against FP64, unitarity and det = 1 on
+    sampled sites, S(0) = 1 and sum_k corr = 3/V.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(2048)
+ALPHA_MAX, LAMBDA_A = 0.7, 0.02
+
+
+def frob_max(a, b):
+    d = (a - b).reshape(-1, 9)
+    return np.sqrt((abs(d) ** 2).sum(axis=1)).max()
+
+
+def alphas(jw, N):
+    import oracle
+    kw = dict(lambda_L=LAMBDA_A * N, mu0_over_lambda=math.exp(2 * math.pi / (9 * ALPHA_MAX)))
+    return jw.alpha(**kw), oracle.alpha(**kw)
+
+
+def mv_start(c, Ny, g2mu, seed=1, traj=0):
+    U = c.field()
+    c.philox(seed, traj, 0)
+    c.mv_init(U, Ny, 1.0, g2mu, 1e-4 / g2mu)
+    return U
+
+
+def test_correlated_step_N2048_against_oracle(jw, orc):
+    N = 2048
+    print(f"production: N={N} correlated FP64 injected step", flush=True)
+    al_g, al_o = alphas(jw, N)
+    c = jw.Context(N)
+    U = mv_start(c, 4, 0.25)
+    U0 = U.download()
+    z = rng.normal(size=N * N * 16)
+    c.inject(z)
+    c.evolve(U, jw.CORRELATED_NOISE_COUPLING, jw.SIN_KERNEL, 5e-5, al_g, nsteps=1)
+    got = U.download()
+    c.close()
+    ref = orc.langevin_step(U0, z, N, 4, 1, 5e-5, al_o)
+    err = frob_max(got, ref)
+    assert err <= 1e-12, err
+    assert frob_max(ref, U0) > 1e-3
+
+
+def test_fixed_two_steps_N1024_mv_start_against_oracle(jw, orc):
+    N = 1024
+    print(f"production: N={N} fixed coupling, MV start Ny=50, two injected steps", flush=True)
+    c = jw.Context(N)
+    U = mv_start(c, 50, 0.25)
+    U0 = U.download()
+    Uo = U0
+    for s in range(2):
+        z = rng.normal(size=N * N * 16)
+        c.inject(z)
+        c.evolve(U, jw.FIXED_COUPLING, jw.SIN_KERNEL, 1e-4, nsteps=1)
+        Uo = orc.langevin_step(Uo, z, N, 3, 1, 1e-4)
+    got = U.download()
+    c.close()
+    err = frob_max(got, Uo)
+    assert err <= 1e-12, err
+    assert frob_max(Uo, U0) > 1e-3
+
+
+def dipole(c, U, N):
+    S = c.dipole_from_corr(c.measure_corr(U))
+    return S
+
+
+def test_mixed_dipole_N2048_1000_steps(jw):
+    N, steps = 2048, 1000
+    print(f"production: N={N} correlated, mixed vs FP64 S(r) after {steps} steps", flush=True)
+    al_g, _ = alphas(jw, N)
+    c = jw.Context(N)
+    U = mv_start(c, 50, 0.25)
+    U0 = U.download()
+    S = {}
+    for form in ("fp64", "mixed"):
+        F = c.field(9, U0)
+        c.philox(3, 0, 0)
+        if form == "mixed":
+            c.evolve_mixed(F, jw.CORRELATED_NOISE_COUPLING, jw.SIN_KERNEL, 5e-5, al_g, nsteps=steps, reunit_every=10)
+        else:
+            c.evolve(F, jw.CORRELATED_NOISE_COUPLING, jw.SIN_KERNEL, 5e-5, al_g, nsteps=steps)
+        S[form] = dipole(c, F, N)
+        F.close()
+    S0 = dipole(c, U, N)
+    c.close()
+    err = np.abs(S["fp64"] - S["mixed"]).max()
+    print(f"production: max |S_mixed - S_fp64| = {err:.3e}", flush=True)
+    assert err <= 2e-5, err
+    assert np.abs(S["fp64"] - S0).max() > 1e-3       # the evolution moved S(r)
+
+
+def test_uneven_ensemble_500_steps(jw):
+    groups = [(256, 48), (512, 32), (1024, 16)]
+    steps = 500
+    print(f"production: ensemble {groups}, {steps} steps, FP64 and mixed", flush=True)
+    ctxs = {N: jw.Context(N) for N, _ in groups}
+    starts = []
+    t = 0
+    for N, count in groups:
+        c = ctxs[N]
+        for _ in range(count):
+            U = mv_start(c, 50, 0.25, seed=2000 + t)
+            starts.append((N, U.download()))
+            U.close()
+            t += 1
+    ens = {}
+    finals = {}
+    for fp32 in (True, False):
+        b = jw.Batch(groups, fp32=fp32)
+        for t, (N, U0) in enumerate(starts):
+            F = ctxs[N].field(9, U0)
+            b.set(t, F)
+            F.close()
+        b.philox(4, 0, 0)
+        b.evolve(jw.FIXED_COUPLING, jw.SIN_KERNEL, 1e-4, None, nsteps=steps, reunit_every=10)
+        b.sync()
+        sums = {N: np.zeros(N * N) for N, _ in groups}
+        fin = {}
+        for t, (N, _) in enumerate(starts):
+            F = ctxs[N].field()
+            b.get(t, F)
+            sums[N] += dipole(ctxs[N], F, N)
+            if t in (0, 47, 48, 95):
+                fin[t] = F.download()
+            F.close()
+        layouts = [b.layout(g) for g in range(len(groups))]
+        b.close()
+        ens[fp32] = {N: sums[N] / cnt for N, cnt in groups}
+        finals[fp32] = (fin, layouts)
+    # sampled trajectories against the same trajectory evolved alone (FP64, its key and layout)
+    fin64, lay64 = finals[False]
+    gi = {0: 0, 47: 0, 48: 1, 95: 2}
+    for t, U1 in fin64.items():
+        N, U0 = starts[t]
+        c = ctxs[N]
+        c.set_fused([c.FUSED_ROWMAJOR, None, c.FUSED_INTERLEAVED, None, c.FUSED_BLOCK4, None, None, None,
+                     c.FUSED_BLOCK8][lay64[gi[t]]])
+        F = c.field(9, U0)
+        c.philox(4, t, 0)
+        c.evolve(F, jw.FIXED_COUPLING, jw.SIN_KERNEL, 1e-4, nsteps=steps)
+        err = frob_max(F.download(), U1)
+        F.close()
+        c.set_fused(c.FUSED)
+        assert err <= 1e-12, (t, err)
+        assert frob_max(U1, U0) > 1e-2
+    for N, _ in groups:
+        d = np.abs(ens[True][N] - ens[False][N]).max()
+        print(f"production: ensemble N={N}: max |<S>_mixed - <S>_fp64| = {d:.3e}", flush=True)
+        assert d <= 2e-5, (N, d)
+    for c in ctxs.values():
+        c.close()
+
+
+@pytest.mark.parametrize("steps", [5])
+def test_large_lattice_N8192(jw, steps):
+    N = 8192
+    print(f"production: N={N} MV Ny=100 g2mu a=0.05, correlated, {steps} steps FP64 and mixed", flush=True)
+    al_g, _ = alphas(jw, N)
+    c = jw.Context(N)
+    U = mv_start(c, 100, 0.05)
+    S = {}
+    for form in ("fp64", "mixed"):
+        F = c.field()
+        c.copy(F, U)
+        c.philox(5, 0, 0)
+        if form == "mixed":
+            c.evolve_mixed(F, jw.CORRELATED_NOISE_COUPLING, jw.SIN_KERNEL, 5e-5, al_g, nsteps=steps, reunit_every=10)
+        else:
+            c.evolve(F, jw.CORRELATED_NOISE_COUPLING, jw.SIN_KERNEL, 5e-5, al_g, nsteps=steps)
+        corr = c.measure_corr(F)
+        assert abs(corr.sum() * (N * N) / 3.0 - 1.0) < 1e-10        # Parseval: sum_k corr = 3/V
+        S[form] = c.dipole_from_corr(corr)
+        assert abs(S[form][0] - 1.0) < 1e-10
+        if form == "fp64":
+            u = F.download().reshape(-1, 3, 3)
+            m = u[rng.integers(0, N * N, 200000)]
+            del u
+            assert np.abs(np.einsum("sij,skj->sik", m, m.conj()) - np.eye(3)).max() < 1e-12
+            assert np.abs(np.linalg.det(m) - 1).max() < 1e-12
+        F.close()
+    S0 = c.dipole_from_corr(c.measure_corr(U))
+    c.close()
+    err = np.abs(S["fp64"] - S["mixed"]).max()
+    print(f"production: N={N} max |S_mixed - S_fp64| = {err:.3e}, moved {np.abs(S['fp64'] - S0).max():.3e}",
+          flush=True)
+    assert err <= 2e-5, err
+    assert np.abs(S["fp64"] - S0).max() > 1e-6

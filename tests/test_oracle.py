@@ -250,6 +250,85 @@ def test_langevin_step_matches_reference_composition(orc, ref, coupling):
         U = a
 
 
+# ---------------------------------------------------------------- correlated noise (coupling 4)
+def _sqrt_alpha_from_reference(ref):
+    """sqrt(alpha(phat2(k))) from the reference's own setCorrelationsForCouplingConstant
+    (field.h:1633-1651), which stores alpha(k) / sqrt(V)."""
+    c = ref.correlations()[:, 0]
+    assert np.abs(c.imag).max() == 0.0
+    return np.sqrt(c.real * N64)
+
+
+def test_correlated_kernel_equals_sqrt_coupling_kernel(orc, ref):
+    # K sqrt(alpha) applied to F xi (coupling 4: noise scaled per mode, kernel factor 1)
+    # is the SQRT coupling's kernel sqrt(alpha) K (field.h:2045-2053) on the same F xi
+    N = N64
+    xx, xy = orc.generate_gaussian(N, rng.normal(size=N * N * 16))
+    fx, fy = npfft(xx, N, 1), npfft(xy, N, 1)
+    a_corr = orc.prepare_A_local(orc.scale_by_sqrt_alpha(fx, N), orc.scale_by_sqrt_alpha(fy, N), N,
+                                 O_mod.CORRELATED, O_mod.SIN)
+    a_sqrt = ref.prepare_A_local(fx, fy, O_mod.SQRT, O_mod.SIN)
+    assert np.abs(a_corr - a_sqrt).max() <= 1e-14 * np.abs(a_sqrt).max()
+    # the per-mode factor is the square root of the reference's alpha(k)
+    s = _sqrt_alpha_from_reference(ref)
+    assert np.allclose(orc.scale_by_sqrt_alpha(np.ones((N * N, 1)), N)[:, 0].real, s, rtol=1e-15, atol=0)
+
+
+def test_correlated_noise_covariance_is_reference_cholesky_target(orc, ref):
+    """The oracle's map T: xi -> xi~ = F^-1[sqrt(alpha) F xi] (one colour component; the map
+    is colour-blind) satisfies T T^T = M, M the dense position-space covariance whose Cholesky
+    factor the reference builds: alpha(k)/sqrt(V) (field.h:1633-1651), transformed backward in
+    the unitary convention (main_optimized.cpp:145-151), laid out as M[i, j] =
+    C(|xi - xj|, |yi - yj|) (matrix.h:151-200)."""
+    N = N64
+    V = N * N
+    x = np.arange(V) // N
+    y = np.arange(V) % N
+
+    def probe(j):
+        f = np.zeros((V, 1), complex)
+        f[j] = 1.0
+        t = orc.correlate_noise(f, N)[:, 0]
+        assert np.abs(t.imag).max() < 1e-16
+        return t.real
+
+    t0 = probe(0)
+    # T is circulant (translation invariant): column j is column 0 shifted by j
+    for j in (1, 77, 2080, V - 1):
+        shift = ((x - x[j]) % N) * N + (y - y[j]) % N
+        assert np.abs(probe(j) - t0[shift]).max() < 1e-15
+    T = t0[((x[:, None] - x[None, :]) % N) * N + (y[:, None] - y[None, :]) % N]
+    K = T @ T.T
+    C = npfft(ref.correlations(), N, 0)[:, 0]
+    assert np.abs(C.imag).max() < 1e-16
+    M = C.real[np.abs(x[:, None] - x[None, :]) * N + np.abs(y[:, None] - y[None, :])]
+    assert np.abs(K - M).max() < 1e-13
+    # a mistake of alpha for sqrt(alpha) gives a visibly different covariance
+    assert np.abs((T @ T) @ (T @ T).T - M).max() > 1e-3
+
+
+def test_correlated_step_matches_reference_composition(orc, ref):
+    """or_langevin_step, coupling 4, against main_optimized.cpp:243-304 composed from the
+    reference's per-site code with the per-mode correlated noise: A with the SQRT kernel,
+    xi~ = F^-1[sqrt(alpha) F xi] with sqrt(alpha) from the reference's own
+    setCorrelationsForCouplingConstant, V = U xi~ U^dagger, B with kernel factor 1."""
+    N = N64
+    s = _sqrt_alpha_from_reference(ref)[:, None]
+    U = random_su3_field(orc, N)
+    for _ in range(2):
+        z = rng.normal(size=N * N * 16)
+        xx, xy = orc.generate_gaussian(N, z)
+        fx, fy = npfft(xx, N, 1), npfft(xy, N, 1)
+        A = npfft(ref.prepare_A_local(fx, fy, O_mod.SQRT, O_mod.SIN), N, 0)
+        vx, vy = ref.uxiulocal(U, npfft(s * fx, N, 0), npfft(s * fy, N, 0))
+        B = npfft(ref.prepare_A_local(npfft(vx, N, 1), npfft(vy, N, 1), O_mod.NOISE, O_mod.SIN), N, 0)
+        expect = ref.update_uf(U, B, A, 5e-5)
+        got = orc.langevin_step(U, z, N, O_mod.CORRELATED, O_mod.SIN, 5e-5)
+        assert np.abs(got - expect).max() < 1e-12
+        assert np.abs(got - U).max() > 1e-4
+        U = got
+
+
 def test_mv_init_matches_reference_composition(orc, ref):
     N, Ny, g, mu, mass = N64, 3, 1.0, 30.72 / 64, 0.5
     z = rng.normal(size=Ny * N * N * 8)
