@@ -81,7 +81,8 @@ int jw_lattice_size(const jw_ctx* ctx);
 const char* jw_last_error(void);
 const char* jw_version(void);
 
-/* ---- fields: lfield<double,t>(Nxl, Nyl) (field.h:152-299) ------------------ */
+/* ---- fields: lfield<double,t>(Nxl, Nyl) (field.h:152-299) ------------------
+ * A field belongs to its context: destroy the fields before the context. */
 int jw_field_create(jw_ctx* ctx, int t, jw_field** out);          /* zero-filled (field.h:70-80) */
 void jw_field_destroy(jw_field* f);
 int jw_field_upload(jw_field* f, const double* host_aos);          /* 2*t*V doubles */
@@ -102,6 +103,18 @@ int jw_noise_inject(jw_ctx* ctx, const double* host_normals, int per_site);
 int jw_noise_philox(jw_ctx* ctx, uint32_t seed, uint32_t traj, uint32_t step);
 int jw_noise_get_step(const jw_ctx* ctx, uint32_t* step);
 
+/* ---- checkpoints ----------------------------------------------------------
+ * No counterpart in the reference (its clock-seeded noise cannot be resumed).
+ * A checkpoint holds U (FP64, the reference's AoS entry order) and the Philox
+ * state {seed, traj, step} of the context; reading it restores both, so that
+ * the evolution continues bit for bit as if it had not stopped.  Format: the
+ * 8 bytes "JWCKPT01", six uint32 (N, seed, traj, step, 9, 0), then 18 V
+ * doubles.  Reading into a context of another N is JW_ERR_ARG; writing while
+ * a noise injection is pending is JW_ERR_STATE.  Both calls synchronise the
+ * context's stream. */
+int jw_checkpoint_write(jw_ctx* ctx, const jw_field* U, const char* path);
+int jw_checkpoint_read(jw_ctx* ctx, jw_field* U, const char* path);
+
 /* ---- reference per-function entry points (9-component fields) ------------ */
 /* fftw2D::execute2D (mpi_fftw_class.h:191-315): unitary, dir 1 = FFTW_FORWARD. */
 int jw_execute2D(jw_ctx* ctx, jw_field* f, int dir);
@@ -118,10 +131,39 @@ int jw_uxiulocal(jw_ctx* ctx, jw_field* v_x, jw_field* v_y, const jw_field* U,
                  const jw_field* xi_x, const jw_field* xi_y);
 /* update_uf (field.h:1966-1996) */
 int jw_update_uf(jw_ctx* ctx, jw_field* U, const jw_field* B, const jw_field* A, double step);
+/* ---- POSITION_EVOLUTION (main_optimized.cpp:271-297) ---------------------
+ * prepare_A_and_B_local (field.h:2368-2519) at every site, the loop of
+ * main_optimized.cpp:288-297: brute-force position-space sums over all site
+ * pairs, A(x) = sum_y w(x-y).(xi_x(y), xi_y(y)) and B(x) the same over
+ * U(y) xi(y) U(y)^dagger, with the kernel of positions.h:131-141 (SIN) or the
+ * integer distances (LINEAR).  coupling: SQRT (the position-space formula of
+ * field.h:2449-2456), NOISE, FIXED or CORRELATED (factor 1: for NOISE the
+ * reference correlates the noise first, main_optimized.cpp:277-282, which
+ * jw_correlate_noise replaces); HATTA is JW_ERR_UNSUPPORTED.  O(V^2) work:
+ * N <= JW_POSITION_MAX_N, a cross-check of the momentum path at small N.
+ * Synchronous. */
+#define JW_POSITION_MAX_N 256
+int jw_prepare_A_and_B_local(jw_ctx* ctx, jw_field* A, jw_field* B, const jw_field* xi_x, const jw_field* xi_y,
+                             const jw_field* U, int coupling, int kernel);
+/* gfield::multiplyByCholesky (field.h:1653-1689) with the factor of
+ * main_optimized.cpp:145-157 (dense Cholesky of the coupling covariance,
+ * matrix.h:151-200), replaced in distribution by the per-mode factor: in place
+ * f <- F^-1[sqrt(alpha(k)) F f] on every component.  Its covariance is the
+ * matrix the reference factors (alpha(k)/sqrt(V) of field.h:1633-1651,
+ * transformed back); the sample-wise factor is the symmetric square root
+ * instead of the Cholesky one. */
+int jw_correlate_noise(jw_ctx* ctx, jw_field* f, const jw_alpha* alpha);
+
 /* lfield::setMVModel (field.h:683-757); mu is MV_class::mu_parameter */
 int jw_set_mv_model(jw_ctx* ctx, jw_field* f, double g, double mu, int Ny);
 /* lfield::solvePoisson (field.h:867-877) */
 int jw_solve_poisson(jw_ctx* ctx, jw_field* f, double mass, double g);
+/* The k = 0 mode of solvePoisson, for jw_solve_poisson and jw_mv_init on this
+ * context.  mode 0 (default): the reference's factor -g/mass^2 with no guard
+ * (field.h:867-877), which at m a = 1e-4 amplifies the zero mode of each
+ * sheet's charge by about 1e8; mode 1: that mode is set to zero (each sheet's
+ * A loses its constant colour part).  No counterpart in the reference. */
+int jw_set_mv_zero_mode(jw_ctx* ctx, int mode);
 /* lfield::operator*= : U <- U exp(i f) (field.h:337-365) */
 int jw_mul_exp(jw_ctx* ctx, jw_field* U, const jw_field* f);
 /* lfield::trace (field.h:1375-1432): corr (t = 1) = tr[(U/V)(U/V)^dagger] */
@@ -167,7 +209,7 @@ int jw_evolve_mixed(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double s
  * Trajectory t draws the Philox stream of key {seed, traj0 + t}, so it is the
  * trajectory a single lattice evolves after jw_noise_philox(seed, traj0 + t,
  * step).  The intermediate-plane layout of each group is chosen on first use
- * by timing each of layouts 0, 2, 4 and 8 on a copy of its state
+ * by timing each of layouts 0, 2, 4, 8 and 16 on a copy of its state
  * (jw_batch_set_layout overrides). */
 typedef struct jw_batch jw_batch;
 int jw_batch_create(jw_batch** out, int device, int ngroups, const int* N, const int* count, int fp32);
@@ -178,7 +220,7 @@ int jw_batch_set(jw_batch* b, int t, const jw_field* U);
 int jw_batch_get(jw_batch* b, int t, jw_field* U);
 /* noise key and step of every trajectory (trajectory t: key {seed, traj0 + t}) */
 int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step);
-/* layout 0, 2, 4 or 8 for group g, or -1 to tune on the next evolution; the chosen one */
+/* layout 0, 2, 4, 8 or 16 for group g, or -1 to tune on the next evolution; the chosen one */
 int jw_batch_set_layout(jw_batch* b, int group, int layout);
 int jw_batch_layout(const jw_batch* b, int group);
 /* nsteps steps of every trajectory (SIN kernel); reunit_every as jw_evolve_mixed
@@ -216,8 +258,18 @@ int jw_saturation_scale(int nbins, const double* rbar, const double* Sbin, const
  * or 8-row-blocked intermediate planes; 5: row-major planes with the column
  * pass taking one plane at a time over 64-byte column strips; 6: the two
  * directions of each plane pair interleaved site by site, so that a column
- * strip of both is one row segment (2..6 for measurement). */
+ * strip of both is one row segment; 7: row-major planes with the column pass
+ * taking one plane at a time over 32-byte column strips (two workgroups per CU
+ * in FP64) (2..7 for measurement). */
 int jw_set_fused(jw_ctx* ctx, int mode);
+/* The column transforms of the mixed-precision form (jw_evolve_mixed, fused
+ * passes, row-major layout).  mode 0 (default): FP32 radix-8/4/2 butterflies;
+ * 1: the radix-16 stages as DFT-matrix products on the matrix cores in split
+ * BF16 (x = hi + lo, three products lo.hi + hi.lo + hi.hi accumulated in FP32,
+ * v_mfma_f32_16x16x32_bf16), the remaining stage in FP32 butterflies: about
+ * 2^-17 relative per matrix-core stage instead of 2^-24.  No counterpart in
+ * the reference (FP64 FFTW). */
+int jw_set_mixed_fft(jw_ctx* ctx, int mode);
 /* Enable per-kernel timing with HIP events on the context's stream. */
 int jw_timing(jw_ctx* ctx, int enable);
 /* Average duration (ms) and call count of kernel class `which` (see jw_kernel_name). */

@@ -161,6 +161,17 @@ class Oracle:
         self._ok(self.L.or_uxiulocal(N, _p(Vx), _p(Vy), _p(U), _p(xix), _p(xiy)))
         return Vx, Vy
 
+    def position_weights(self, N, coupling, kernel):
+        wa, wb = np.zeros(N * N), np.zeros(N * N)
+        self._ok(self.L.or_position_weights(N, coupling, kernel, _p(wa), _p(wb)))
+        return wa, wb
+
+    def prepare_A_and_B_local(self, xix, xiy, U, N, coupling, kernel):
+        A, B = _field(N), _field(N)
+        xix, xiy, U = (np.ascontiguousarray(a, dtype=np.complex128) for a in (xix, xiy, U))
+        self._ok(self.L.or_prepare_A_and_B_local(N, _p(A), _p(B), _p(xix), _p(xiy), _p(U), coupling, kernel))
+        return A, B
+
     def update_uf(self, U, B, A, step, N):
         U = np.ascontiguousarray(U, dtype=np.complex128).copy()
         B, A = (np.ascontiguousarray(a, dtype=np.complex128) for a in (B, A))
@@ -204,11 +215,11 @@ class Oracle:
                                          ctypes.byref(al) if al else None))
         return U
 
-    def mv_init(self, N, z, Ny, g, mu, mass):
+    def mv_init(self, N, z, Ny, g, mu, mass, zero_mode=0):
         U = _field(N)
         z = np.ascontiguousarray(z, dtype=np.float64)
-        self._ok(self.L.or_mv_init(N, _p(U), _p(z), Ny, ctypes.c_double(g), ctypes.c_double(mu),
-                                   ctypes.c_double(mass)))
+        self._ok(self.L.or_mv_init_opt(N, _p(U), _p(z), Ny, ctypes.c_double(g), ctypes.c_double(mu),
+                                       ctypes.c_double(mass), int(zero_mode)))
         return U
 
     def measure_corr(self, U, N):
@@ -290,6 +301,12 @@ class Reference:
         U, xix, xiy = (np.ascontiguousarray(a, dtype=np.complex128) for a in (U, xix, xiy))
         self.L.ref_uxiulocal(_p(Vx), _p(Vy), _p(U), _p(xix), _p(xiy))
         return Vx, Vy
+
+    def prepare_A_and_B_local(self, xix, xiy, U, coupling, kernel):
+        A, B = _field(self.N), _field(self.N)
+        xix, xiy, U = (np.ascontiguousarray(a, dtype=np.complex128) for a in (xix, xiy, U))
+        assert self.L.ref_prepare_A_and_B_local(_p(A), _p(B), _p(xix), _p(xiy), _p(U), coupling, kernel) == 0
+        return A, B
 
     def update_uf(self, U, B, A, step):
         U = np.ascontiguousarray(U, dtype=np.complex128).copy()

@@ -739,7 +739,10 @@ int or_langevin_step(int N, double* U, const double* z, int coupling, int kernel
 // MV initial condition, main_optimized.cpp:178-191: U = 1; per sheet
 // f = setMVModel; F f; solvePoisson(mass*g^2*mu, g); F^-1 f; U *= f.
 // z: Ny * V * 8 standard normals (sheet-major).  mu is MV_class's mu_parameter.
-int or_mv_init(int N, double* U, const double* z, int Ny, double g, double mu, double mass) {
+// zero_mode = 1: each sheet's k = 0 mode is set to zero after solvePoisson
+// instead of carrying the factor -g/m^2 (the library's projection option; the
+// reference has no such switch)
+int or_mv_init_opt(int N, double* U, const double* z, int Ny, double g, double mu, double mass, int zero_mode) {
     if (!pow2(N) || Ny < 1) return -1;
     const size_t V = (size_t)N * N;
     std::vector<double> f(18 * V);
@@ -749,10 +752,16 @@ int or_mv_init(int N, double* U, const double* z, int Ny, double g, double mu, d
         or_set_mv_model(N, z + (size_t)s * V * 8, sigma, f.data());
         or_fft2d(N, 9, f.data(), 1);
         or_solve_poisson(N, f.data(), mass * std::pow(g, 2.0) * mu, g);
+        if (zero_mode)
+            for (int k = 0; k < 18; k++) f[k] = 0.0;
         or_fft2d(N, 9, f.data(), 0);
         or_mul_exp(N, U, f.data());
     }
     return 0;
+}
+
+int or_mv_init(int N, double* U, const double* z, int Ny, double g, double mu, double mass) {
+    return or_mv_init_opt(N, U, z, Ny, g, mu, mass, 0);
 }
 
 // Measurement, main_optimized.cpp:321-333 (single rank): copy, F U, trace,
@@ -763,6 +772,89 @@ int or_measure_corr(int N, const double* U, double* corr) {
     or_fft2d(N, 9, uc.data(), 1);
     or_trace(N, uc.data(), corr);
     return or_average_and_symmetrize(N, corr);
+}
+
+// ---- POSITION_EVOLUTION (main_optimized.cpp:271-297, field.h:2368-2519) ----
+// Kernel weights of the position-space step for every displacement
+// r = (rx, ry) = (x - xx, y - yy) mod N, at index rx N + ry:
+//   SIN kernel (positions.h:131-141 tables): dx = 0.5 N sin(2 pi rx/N)/pi,
+//     dy likewise, rrr = pow(N sin(pi rx/N)/pi, 2) + pow(N sin(pi ry/N)/pi, 2);
+//   LINEAR kernel: dx, dy = rx, ry wrapped into [-N/2, N/2), rrr = dx^2 + dy^2;
+//   SQRT coupling (0): c = sqrt(4 pi / ((11 - 2*3/3) log(pow(lambda + 1.26 /
+//     pow(36 rrr/N^2, 1/0.2), 0.2)))) with lambda = pow(225/36, 1/0.2)
+//     (field.h:2449-2456); NOISE (1) and FIXED (3): c = 1 (the reference applies
+//     its Cholesky factor to the noise of NOISE beforehand, main_optimized.cpp:277-282);
+//   wa = c dx / rrr, wb = c dy / rrr where rrr > 10e-9, else 0 (field.h:2460-2468).
+int or_position_weights(int N, int coupling, int kernel, double* wa, double* wb) {
+    if (!pow2(N) || (coupling != 0 && coupling != 1 && coupling != 3) || (kernel != 0 && kernel != 1)) return -1;
+    const double lambda = std::pow(15.0 * 15.0 / 6.0 / 6.0, 1.0 / 0.2);
+    for (int rx = 0; rx < N; rx++)
+        for (int ry = 0; ry < N; ry++) {
+            double dx, dy, rrr;
+            if (kernel == 1) {
+                dx = 0.5 * N * std::sin(2.0 * M_PI * rx / (1.0 * N)) / M_PI;
+                dy = 0.5 * N * std::sin(2.0 * M_PI * ry / (1.0 * N)) / M_PI;
+                const double bx = N * std::sin(M_PI * rx / (1.0 * N)) / M_PI;
+                const double by = N * std::sin(M_PI * ry / (1.0 * N)) / M_PI;
+                rrr = std::pow(bx, 2.0) + std::pow(by, 2.0);
+            } else {
+                dx = rx >= N / 2 ? rx - N : rx;
+                dy = ry >= N / 2 ? ry - N : ry;
+                rrr = 1.0 * (dx * dx + dy * dy);
+            }
+            double c = 1.0;
+            if (coupling == 0)
+                c = std::sqrt(4.0 * M_PI /
+                              ((11.0 - 2.0 * 3.0 / 3.0) *
+                               std::log(std::pow(lambda + 1.26 / std::pow(6.0 * 6.0 * rrr / N / N, 1.0 / 0.2), 0.2))));
+            const size_t i = (size_t)rx * N + ry;
+            wa[i] = rrr > 10e-9 ? c * dx / rrr : 0.0;
+            wb[i] = rrr > 10e-9 ? c * dy / rrr : 0.0;
+        }
+    return 0;
+}
+
+// prepare_A_and_B_local for every site (the loop of main_optimized.cpp:288-297):
+//   A(x) = sum_y E_xy,  B(x) = sum_y U(y) E_xy U(y)^dagger,
+//   E_xy = wa(x - y) xi_x(y) + wb(x - y) xi_y(y)
+// with the products in the reference's order (field.h:2478-2506): C = A xi_x,
+// D = B xi_y entry by entry, E = C + D, K = (G E) H with H = G^dagger; sums
+// over y in row-major order (the reference's OpenMP reduction has no fixed order).
+int or_prepare_A_and_B_local(int N, double* A, double* B, const double* xix, const double* xiy, const double* U,
+                             int coupling, int kernel) {
+    const size_t V = (size_t)N * N;
+    std::vector<double> wa(V), wb(V);
+    if (or_position_weights(N, coupling, kernel, wa.data(), wb.data())) return -1;
+    const cd* X = C(xix);
+    const cd* Y = C(xiy);
+    const cd* G = C(U);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (size_t t = 0; t < V; t++) {
+        const int x = (int)(t / N), y = (int)(t % N);
+        cd sa[9], sb[9];
+        for (int k = 0; k < 9; k++) sa[k] = sb[k] = 0.0;
+        for (int xx = 0; xx < N; xx++)
+            for (int yy = 0; yy < N; yy++) {
+                const size_t i = (size_t)xx * N + yy;
+                const size_t r = (size_t)((x - xx + N) % N) * N + (size_t)((y - yy + N) % N);
+                const cd a(wa[r], 0.0), b(wb[r], 0.0);
+                M3 E, Gm;
+                for (int k = 0; k < 9; k++) {
+                    E.m[k] = a * X[9 * i + k] + b * Y[9 * i + k];
+                    Gm.m[k] = G[9 * i + k];
+                }
+                const M3 K = mul(mul(Gm, E), dagger(Gm));
+                for (int k = 0; k < 9; k++) {
+                    sa[k] += E.m[k];
+                    sb[k] += K.m[k];
+                }
+            }
+        for (int k = 0; k < 9; k++) {
+            C(A)[9 * t + k] = sa[k];
+            C(B)[9 * t + k] = sb[k];
+        }
+    }
+    return 0;
 }
 
 // ---- dipole correlator (beyond the reference: derived by Wiener-Khinchin) ----

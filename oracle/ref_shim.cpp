@@ -190,6 +190,37 @@ int ref_correlations(double* corr) {
     return 0;
 }
 
+positions* single_positions() {
+    static positions* pos = nullptr;
+    if (!pos) {
+        pos = new positions(single_config(), single_rank());
+        pos->set();
+    }
+    return pos;
+}
+
+// prepare_A_and_B_local (field.h:2368-2519) for every site, the loop of
+// main_optimized.cpp:288-297 (coupling: 0 SQRT, 1 NOISE; kernel: 0 LINEAR, 1 SIN)
+int ref_prepare_A_and_B_local(double* A, double* B, const double* xix, const double* xiy, const double* U,
+                              int coupling, int kernel) {
+    if (coupling != 0 && coupling != 1) return -1;
+    gfield<double, 9> x(Nx, Ny), y(Nx, Ny), u(Nx, Ny);
+    lfield<double, 9> a(Nx, Ny), b(Nx, Ny);
+    std::memcpy((void*)x.u, xix, sizeof(std::complex<double>) * 9 * Nx * Ny);
+    std::memcpy((void*)y.u, xiy, sizeof(std::complex<double>) * 9 * Nx * Ny);
+    std::memcpy((void*)u.u, U, sizeof(std::complex<double>) * 9 * Nx * Ny);
+    a.setToZero();
+    b.setToZero();
+    for (int xx = 0; xx < Nx; xx++)
+        for (int yy = 0; yy < Ny; yy++)
+            prepare_A_and_B_local(xx, yy, xx, yy, &x, &y, &a, &b, &u, single_positions(), 0,
+                                  coupling == 0 ? SQRT_COUPLING_CONSTANT : NOISE_COUPLING_CONSTANT,
+                                  kernel == 0 ? LINEAR_KERNEL : SIN_KERNEL);
+    store(a, A);
+    store(b, B);
+    return 0;
+}
+
 // gfield::reduce(sum, err, mpi), field.h:1521-1536
 int ref_reduce(const double* corr, double* sum, double* err) {
     gfield<double, 1> g(Nx, Ny);

@@ -49,6 +49,12 @@ int ensure_ws(jw_ctx* c, size_t planes) {
     return JW_OK;
 }
 
+int check_alpha(const jw::Alpha& a) {
+    if (!(a.lambda_L > 0) || !(a.mu0_over_lambda > 1.0) || !(a.freeze_c > 0) || !(a.beta0 > 0))
+        return fail(JW_ERR_ARG, "invalid running-coupling parameters");
+    return JW_OK;
+}
+
 int ensure_sa(jw_ctx* c, int variant, const jw::Alpha& a) {
     if (!c->sa) {
         if (hipMalloc(&c->sa, c->V * sizeof(double)) != hipSuccess)
@@ -56,9 +62,8 @@ int ensure_sa(jw_ctx* c, int variant, const jw::Alpha& a) {
         c->sa_variant = -1;
     }
     if (c->sa_variant == variant && memcmp(&c->sa_params, &a, sizeof a) == 0) return JW_OK;
-    if (!(a.lambda_L > 0) || !(a.mu0_over_lambda > 1.0) || !(a.freeze_c > 0) || !(a.beta0 > 0))
-        return fail(JW_ERR_ARG, "invalid running-coupling parameters");
-    HIP_CHECK(jw::build_sqrt_alpha(c, variant, a));
+    if (int e = check_alpha(a)) return e;
+    HIP_CHECK(jw::build_sqrt_alpha(c, c->sa, variant, a));
     c->sa_variant = variant;
     c->sa_params = a;
     c->sa_gen += 1;
@@ -101,7 +106,7 @@ int check_field(const jw_field* f, const jw_ctx* c, int t, const char* name) {
 // row-major 382 / 568, 418 / 653; 4-row blocks 363 / 474, 415 / 536; 8-row
 // blocks 345 / 507, 407 / 559.  Row-major everywhere.
 // the column passes of layouts 1, 4 and 8 read the 16-row-blocked sqrt(alpha) table
-bool uses_sa16(int layout) { return layout == 1 || layout == 4 || layout == 8; }
+bool uses_sa16(int layout) { return layout == 1 || layout == 4 || layout == 8 || layout == 16; }
 
 int default_layout(int N, int corr, int fp32) {
     (void)N;
@@ -129,6 +134,62 @@ int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, jw::c
     return JW_OK;
 }
 
+
+// The generator-basis evolution of jw_evolve for the SIN kernel on the 24
+// intermediate planes at Xi (xi 8, V 8, A 4, B 4).  tabs == nullptr: the
+// context's cached coupling tables (allocated on first use); otherwise two
+// V-double arrays into which this call builds sqrt(alpha) and, for the blocked
+// layouts, its 16-row-blocked copy (nothing allocated).
+int evolve_sin(jw_ctx* c, jw_field* U, int coupling, double step, const jw_alpha* a, int nsteps, cplx* Xi,
+               double* tabs) {
+    const size_t V = c->V;
+    cplx* Vv = Xi + 8 * V;
+    cplx* ZA = Vv + 8 * V;
+    cplx* ZB = ZA + 4 * V;
+    const jw::Alpha al = to_alpha(a);
+    const double* sa_pos = nullptr;
+    const double* sa_k = nullptr;
+    if (coupling == JW_NOISE_COUPLING_CONSTANT || coupling == JW_SQRT_COUPLING_CONSTANT ||
+        coupling == JW_CORRELATED_NOISE_COUPLING) {
+        const int variant = coupling == JW_NOISE_COUPLING_CONSTANT ? 1 : 0;
+        const double* sa;
+        if (tabs) {
+            if (int e = check_alpha(al)) return e;
+            HIP_CHECK(jw::build_sqrt_alpha(c, tabs, variant, al));
+            sa = tabs;
+        } else {
+            if (int e = ensure_sa(c, variant, al)) return e;
+            sa = c->sa;
+        }
+        if (variant == 1) sa_pos = sa;
+        else sa_k = sa;
+    }
+    const int kc = coupling == JW_SQRT_COUPLING_CONSTANT ? 0 : 1;
+    const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
+    const double s = 1.0 / c->N, ss = sqrt(step);
+    const bool fused = jw::fused_supported(c->N) && !c->unfused;
+    const int saved_layout = c->layout;
+    if (c->layout < 0) c->layout = default_layout(c->N, corr, 0);
+    if (fused && sa_k && uses_sa16(c->layout)) {
+        if (tabs) {
+            const hipError_t he = jw::block16(c, tabs + V, tabs);
+            if (he != hipSuccess) {
+                c->layout = saved_layout;
+                return fail(JW_ERR_HIP, "coupling table: %s", hipGetErrorString(he));
+            }
+            sa_k = tabs + V;
+        } else {
+            if (int e = ensure_sa16(c)) {
+                c->layout = saved_layout;
+                return e;
+            }
+            sa_k = c->sa16;
+        }
+    }
+    const int rc = evolve_generator_basis(c, U, nsteps, fused, Xi, Vv, ZA, ZB, sa_pos, sa_k, kc, corr, s, ss);
+    c->layout = saved_layout;
+    return rc;
+}
 
 // ---------------------------------------------------------------- batches
 // One size group of a batch: `count` trajectories of one N, state arrays of
@@ -262,28 +323,28 @@ int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
         BatchGroup& g = b->g[gi];
         if (g.layout >= 0) continue;
         const size_t ubytes = (size_t)g.count * 9 * g.ctx->V * elem_size(b);
-        void* scratch = nullptr;
-        if (hipMalloc(&scratch, ubytes) != hipSuccess) return fail(JW_ERR_ALLOC, "tuning buffer allocation failed");
+        void* ucopy = nullptr;
+        if (hipMalloc(&ucopy, ubytes) != hipSuccess) return fail(JW_ERR_ALLOC, "tuning buffer allocation failed");
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (hipMemcpyAsync(scratch, g.U, ubytes, hipMemcpyDeviceToDevice, g.ctx->stream) != hipSuccess ||
+        if (hipMemcpyAsync(ucopy, g.U, ubytes, hipMemcpyDeviceToDevice, g.ctx->stream) != hipSuccess ||
             hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
             if (e0) hipEventDestroy(e0);
             hipStreamSynchronize(g.ctx->stream);
-            hipFree(scratch);
+            hipFree(ucopy);
             return fail(JW_ERR_HIP, "tuning set-up failed for group %zu", gi);
         }
         const uint32_t saved_traj0 = b->traj0;
         b->traj0 = 0xFFFFF000u;      // scratch noise key
         float best = 1e30f;
         int best_layout = -1;
-        for (int lay : {0, 2, 4, 8}) {
+        for (int lay : {0, 2, 4, 8, 16}) {
             g.ctx->layout = lay;
             const double *sa_pos, *sa_k;
             if (batch_tables(g, coupling, al, &sa_pos, &sa_k) != JW_OK) continue;
-            bool ok = batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, scratch, 1, 0) == hipSuccess;
+            bool ok = batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, ucopy, 1, 0) == hipSuccess;
             ok = ok && hipEventRecord(e0, g.ctx->stream) == hipSuccess;
             for (int it = 0; it < 3 && ok; it++)
-                ok = batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, scratch, 1, 0) == hipSuccess;
+                ok = batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, ucopy, 1, 0) == hipSuccess;
             ok = ok && hipEventRecord(e1, g.ctx->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
             float ms = 0.f;
             ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
@@ -300,7 +361,7 @@ int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
         hipEventDestroy(e1);
         b->traj0 = saved_traj0;
         hipStreamSynchronize(g.ctx->stream);
-        hipFree(scratch);
+        hipFree(ucopy);
         if (best_layout < 0) {
             g.ctx->layout = -1;
             return fail(JW_ERR_HIP, "tuning: no intermediate layout ran for group %zu", gi);
@@ -483,6 +544,51 @@ int jw_noise_get_step(const jw_ctx* c, uint32_t* step) {
     return JW_OK;
 }
 
+// ---------------------------------------------------------------- checkpoints
+// File: 8-byte magic "JWCKPT01", then uint32 N, seed, traj, step (Philox
+// state), uint32 t = 9, uint32 0, then 18 V doubles of U in the reference's AoS
+// order (site x N + y, entry 3 row + col, re and im).
+static const char kCkptMagic[8] = {'J', 'W', 'C', 'K', 'P', 'T', '0', '1'};
+
+int jw_checkpoint_write(jw_ctx* c, const jw_field* U, const char* path) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (!path) return fail(JW_ERR_ARG, "null path");
+    if (c->inj_per_site) return fail(JW_ERR_STATE, "a noise injection is pending: the stream state is not Philox");
+    std::vector<double> h(18 * c->V);
+    if (int e = jw_field_download(U, h.data())) return e;
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(JW_ERR_ARG, "checkpoint %s: cannot open for writing", path);
+    const uint32_t hdr[6] = {(uint32_t)c->N, c->seed, c->traj, c->step, 9u, 0u};
+    const bool ok = fwrite(kCkptMagic, 1, 8, f) == 8 && fwrite(hdr, sizeof hdr, 1, f) == 1 &&
+                    fwrite(h.data(), sizeof(double), h.size(), f) == h.size();
+    if (fclose(f) != 0 || !ok) return fail(JW_ERR_ARG, "checkpoint %s: write failed", path);
+    return JW_OK;
+}
+
+int jw_checkpoint_read(jw_ctx* c, jw_field* U, const char* path) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (!path) return fail(JW_ERR_ARG, "null path");
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(JW_ERR_ARG, "checkpoint %s: cannot open", path);
+    char magic[8];
+    uint32_t hdr[6];
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kCkptMagic, 8) == 0 && fread(hdr, sizeof hdr, 1, f) == 1;
+    if (!ok) {
+        fclose(f);
+        return fail(JW_ERR_ARG, "checkpoint %s: not a checkpoint file", path);
+    }
+    if ((int)hdr[0] != c->N || hdr[4] != 9u) {
+        fclose(f);
+        return fail(JW_ERR_ARG, "checkpoint %s: lattice N = %u, context N = %d", path, hdr[0], c->N);
+    }
+    std::vector<double> h(18 * c->V);
+    ok = fread(h.data(), sizeof(double), h.size(), f) == h.size();
+    fclose(f);
+    if (!ok) return fail(JW_ERR_ARG, "checkpoint %s: truncated", path);
+    if (int e = jw_field_upload(U, h.data())) return e;
+    return jw_noise_philox(c, hdr[1], hdr[2], hdr[3]);
+}
+
 // ---------------------------------------------------------------- per-function entry points
 int jw_execute2D(jw_ctx* c, jw_field* f, int dir) {
     if (int e = check_field(f, c, 0, "field")) return e;
@@ -560,6 +666,82 @@ int jw_set_mv_model(jw_ctx* c, jw_field* f, double g, double mu, int Ny) {
 int jw_solve_poisson(jw_ctx* c, jw_field* f, double mass, double g) {
     if (int e = check_field(f, c, 0, "f")) return e;
     HIP_CHECK(jw::solve_poisson(c, f->d, f->t, mass, g));
+    return JW_OK;
+}
+
+// POSITION_EVOLUTION weights per displacement r = (rx, ry), index rx N + ry:
+// SIN kernel from the positions.h:131-141 tables, LINEAR from the wrapped
+// integer distances, SQRT coupling with field.h:2449-2456's position-space
+// formula, else factor 1; zero where rrr <= 10e-9 (field.h:2460-2468)
+static void position_weights(int N, int coupling, int kernel, double* wa, double* wb) {
+    const double lambda = pow(15.0 * 15.0 / 6.0 / 6.0, 1.0 / 0.2);
+    for (int rx = 0; rx < N; rx++)
+        for (int ry = 0; ry < N; ry++) {
+            double dx, dy, rrr;
+            if (kernel == JW_SIN_KERNEL) {
+                dx = 0.5 * N * sin(2.0 * M_PI * rx / (1.0 * N)) / M_PI;
+                dy = 0.5 * N * sin(2.0 * M_PI * ry / (1.0 * N)) / M_PI;
+                const double bx = N * sin(M_PI * rx / (1.0 * N)) / M_PI;
+                const double by = N * sin(M_PI * ry / (1.0 * N)) / M_PI;
+                rrr = pow(bx, 2.0) + pow(by, 2.0);
+            } else {
+                dx = rx >= N / 2 ? rx - N : rx;
+                dy = ry >= N / 2 ? ry - N : ry;
+                rrr = 1.0 * (dx * dx + dy * dy);
+            }
+            double cc = 1.0;
+            if (coupling == JW_SQRT_COUPLING_CONSTANT)
+                cc = sqrt(4.0 * M_PI /
+                          ((11.0 - 2.0 * 3.0 / 3.0) * log(pow(lambda + 1.26 / pow(6.0 * 6.0 * rrr / N / N, 1.0 / 0.2), 0.2))));
+            const size_t i = (size_t)rx * N + ry;
+            wa[i] = rrr > 10e-9 ? cc * dx / rrr : 0.0;
+            wb[i] = rrr > 10e-9 ? cc * dy / rrr : 0.0;
+        }
+}
+
+int jw_prepare_A_and_B_local(jw_ctx* c, jw_field* A, jw_field* B, const jw_field* x, const jw_field* y,
+                             const jw_field* U, int coupling, int kernel) {
+    if (int e = check_field(A, c, 9, "A")) return e;
+    if (int e = check_field(B, c, 9, "B")) return e;
+    if (int e = check_field(x, c, 9, "xi_x")) return e;
+    if (int e = check_field(y, c, 9, "xi_y")) return e;
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (coupling == JW_HATTA_COUPLING_CONSTANT)
+        return fail(JW_ERR_UNSUPPORTED, "HATTA coupling: the reference leaves the coupling uninitialised here");
+    if (coupling < 0 || coupling > 4 || (kernel != 0 && kernel != 1)) return fail(JW_ERR_ARG, "bad coupling/kernel");
+    if (c->N > JW_POSITION_MAX_N)
+        return fail(JW_ERR_UNSUPPORTED, "N = %d: the O(V^2) position-space sums run up to N = %d", c->N,
+                    JW_POSITION_MAX_N);
+    if (A == B || A == x || A == y || A == U || B == x || B == y || B == U)
+        return fail(JW_ERR_ARG, "A and B must be distinct from the inputs");
+    // workspace: V_x (9 planes), V_y (9), the two weight tables (2 V doubles = 1 plane)
+    if (int e = ensure_ws(c, 19)) return e;
+    const size_t V = c->V;
+    cplx* vx = c->ws;
+    cplx* vy = vx + 9 * V;
+    double* w = reinterpret_cast<double*>(vy + 9 * V);
+    std::vector<double> hw(2 * V);
+    position_weights(c->N, coupling, kernel, hw.data(), hw.data() + V);
+    HIP_CHECK(hipMemcpyAsync(w, hw.data(), 2 * V * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(jw::uxiulocal(c, vx, vy, U->d, x->d, y->d));
+    HIP_CHECK(jw::position_sums(c, A->d, B->d, x->d, y->d, vx, vy, w, w + V));
+    HIP_CHECK(hipStreamSynchronize(c->stream));     // hw is released on return
+    return JW_OK;
+}
+
+int jw_correlate_noise(jw_ctx* c, jw_field* f, const jw_alpha* a) {
+    if (int e = check_field(f, c, 0, "field")) return e;
+    if (int e = ensure_sa(c, 0, to_alpha(a))) return e;
+    HIP_CHECK(jw::fft2d(c, f->d, f->t, 1, 1.0 / c->N));
+    HIP_CHECK(jw::scale_by_table(c, f->d, f->t, c->sa));
+    HIP_CHECK(jw::fft2d(c, f->d, f->t, 0, 1.0 / c->N));
+    return JW_OK;
+}
+
+int jw_set_mv_zero_mode(jw_ctx* c, int mode) {
+    if (!c) return fail(JW_ERR_ARG, "null context");
+    if (mode != 0 && mode != 1) return fail(JW_ERR_ARG, "zero mode %d: 0 (reference) or 1 (projected out)", mode);
+    c->mv_zero_mode = mode;
     return JW_OK;
 }
 
@@ -644,33 +826,7 @@ int jw_evolve(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, con
         return JW_OK;
     }
     if (int e = ensure_ws(c, 24)) return e;
-    const size_t V = c->V;
-    cplx* Xi = c->ws;
-    cplx* Vv = Xi + 8 * V;
-    cplx* ZA = Vv + 8 * V;
-    cplx* ZB = ZA + 4 * V;
-    const double* sa_pos = nullptr;
-    const double* sa_k = nullptr;
-    if (coupling == JW_NOISE_COUPLING_CONSTANT) {
-        if (int e = ensure_sa(c, 1, to_alpha(a))) return e;
-        sa_pos = c->sa;
-    } else if (coupling == JW_SQRT_COUPLING_CONSTANT || coupling == JW_CORRELATED_NOISE_COUPLING) {
-        if (int e = ensure_sa(c, 0, to_alpha(a))) return e;
-        sa_k = c->sa;
-    }
-    const int kc = coupling == JW_SQRT_COUPLING_CONSTANT ? 0 : 1;
-    const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
-    const double s = 1.0 / c->N, ss = sqrt(step);
-    const bool fused = jw::fused_supported(c->N) && !c->unfused;
-    const int saved_layout = c->layout;
-    if (c->layout < 0) c->layout = default_layout(c->N, corr, 0);
-    if (fused && sa_k && uses_sa16(c->layout)) {
-        if (int e = ensure_sa16(c)) { c->layout = saved_layout; return e; }
-        sa_k = c->sa16;
-    }
-    const int rc = evolve_generator_basis(c, U, nsteps, fused, Xi, Vv, ZA, ZB, sa_pos, sa_k, kc, corr, s, ss);
-    c->layout = saved_layout;
-    return rc;
+    return evolve_sin(c, U, coupling, step, a, nsteps, c->ws, nullptr);
 }
 
 int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, const jw_alpha* a, int nsteps,
@@ -839,10 +995,17 @@ static const char* kNames[] = {"fft_rows", "fft_cols", "gen_noise", "kernel_mult
 
 int jw_set_fused(jw_ctx* c, int mode) {
     if (!c) return fail(JW_ERR_ARG, "null context");
-    static const int layouts[7] = {0, JW_DEFAULT_LAYOUT, 0, 4, 8, 1, 2};
-    if (mode < 0 || mode > 6) return fail(JW_ERR_ARG, "fused mode %d: expected 0..6", mode);
+    static const int layouts[8] = {0, JW_DEFAULT_LAYOUT, 0, 4, 8, 1, 2, 16};
+    if (mode < 0 || mode > 7) return fail(JW_ERR_ARG, "fused mode %d: expected 0..7", mode);
     c->unfused = mode ? 0 : 1;
     c->layout = layouts[mode];
+    return JW_OK;
+}
+
+int jw_set_mixed_fft(jw_ctx* c, int mode) {
+    if (!c) return fail(JW_ERR_ARG, "null context");
+    if (mode != 0 && mode != 1) return fail(JW_ERR_ARG, "mixed FFT mode %d: 0 (FP32 butterflies) or 1 (matrix cores)", mode);
+    c->fft_mfma = mode;
     return JW_OK;
 }
 
@@ -1002,7 +1165,7 @@ int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step) {
 
 int jw_batch_set_layout(jw_batch* b, int group, int layout) {
     if (!b || group < 0 || group >= (int)b->g.size()) return fail(JW_ERR_ARG, "bad batch group");
-    if (layout != -1 && layout != 0 && layout != 2 && layout != 4 && layout != 8)
+    if (layout != -1 && layout != 0 && layout != 2 && layout != 4 && layout != 8 && layout != 16)
         return fail(JW_ERR_ARG, "layout %d", layout);
     HIP_CHECK(hipStreamSynchronize(b->stream));
     b->g[group].layout = layout;

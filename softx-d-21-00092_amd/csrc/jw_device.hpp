@@ -149,6 +149,8 @@ __device__ __forceinline__ void gen_from_upper(const m3& M, double h[8]) {
 #define JW_EXP_SERIES_TERMS 24
 #define JW_EXP_SERIES_TAIL 1e-17
 
+__device__ __forceinline__ m3 su3_project(const m3& M);
+
 __device__ __forceinline__ m3 exp_i_gen(const double h[8], double s) {
     double c1 = 0.0;
 #pragma unroll
@@ -197,6 +199,10 @@ __device__ __forceinline__ m3 exp_i_gen(const double h[8], double s) {
     E.m[4] = cadd(E.m[4], f0);
     E.m[8] = cadd(E.m[8], f0);
     for (int j = 0; j < k; j++) E = mmul(E, E);
+    // k squarings multiply the rounding of E by up to 2^k; the part that breaks
+    // unitarity is taken out again (an MV sheet at m a = 1e-4 carries a zero
+    // mode of norm ~1e4: k ~ 15, |E^dagger E - 1| ~ 1e-11 without this)
+    if (k > 0) E = su3_project(E);
     return E;
 }
 
@@ -473,26 +479,151 @@ __device__ __forceinline__ void fft_stage(C* buf, TW tw, int tid) {
     __syncthreads();
 }
 
-template <int N, int P, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
+// ---------------------------------------------------------------------------
+// Split-BF16 radix-16 stage on the matrix cores (FP32 data, the mixed form).
+// The 16 twiddled inputs of a butterfly form one column of the 32 x 16 real
+// matrix B = [Re v; Im v] (16 butterflies per wave); the radix-16 DFT is the
+// real block matrix [[Fr, -Fi], [Fi, Fr]], F[i][r] = exp(DIR 2 pi i i r / 16),
+// applied as its real-output half A_re = [Fr | -Fi] and imaginary-output half
+// A_im = [Fi | Fr] (16 x 32 each) by v_mfma_f32_16x16x32_bf16.  Every FP32
+// operand x is split into hi = bf16(x) and lo = bf16(x - hi); the products
+// lo.hi + hi.lo, then hi.hi, accumulate in FP32 (the dropped lo.lo and the
+// split leave about 2^-17 relative per stage, against 2^-24 for the FP32
+// butterflies).  Lane l of the 16x16x32 instruction (wave64) holds A row
+// l mod 16, columns 8 (l / 16) + j; B rows 8 (l / 16) + j, column l mod 16;
+// D rows 4 (l / 16) + r (r < 4), column l mod 16.
+// ---------------------------------------------------------------------------
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+struct Dft16Mfma {
+    bf16x8 re_hi, re_lo, im_hi, im_lo;
+};
+
+// cos(2 pi m / 16), m = 0..15
+__device__ __forceinline__ float cos16(int m) {
+    const float c1 = 0.92387953251128675613f, c2 = 0.70710678118654752440f, c3 = 0.38268343236508977173f;
+    const float t[16] = {1.f, c1, c2, c3, 0.f, -c3, -c2, -c1, -1.f, -c1, -c2, -c3, 0.f, c3, c2, c1};
+    return t[m & 15];
+}
+
+// element t of (hi, lo) <- the split of x
+__device__ __forceinline__ void split_bf16(float x, bf16x8& hi, bf16x8& lo, int t) {
+    const __bf16 h = (__bf16)x;
+    hi[t] = h;
+    lo[t] = (__bf16)(x - (float)h);
+}
+
+// this lane's slice of A_re and A_im for direction DIR
+template <int DIR>
+__device__ __forceinline__ Dft16Mfma dft16_operands(int lane) {
+    const int i = lane & 15, kb = lane >> 4;
+    Dft16Mfma a;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int k = 8 * kb + j, r = k & 15, m = (i * r) & 15;
+        const float c = cos16(m), s = DIR * cos16(m + 12);       // sin(x) = cos(x - pi/2)
+        split_bf16(k < 16 ? c : -s, a.re_hi, a.re_lo, j);
+        split_bf16(k < 16 ? s : c, a.im_hi, a.im_lo, j);
+    }
+    return a;
+}
+
+// One radix-16 Stockham stage (the indexing of fft_stage with R = 16) on the
+// matrix cores: butterflies in groups of 16, one group per wave at a time;
+// reads, a barrier, MFMAs and writes, a barrier.
+template <int N, int P, int NSEQ, int ES, int SS, int NT, int DIR, int PS, typename TW>
+__device__ __forceinline__ void fft_stage_mfma16(float2* buf, TW tw, int tid) {
+    constexpr int R = 16, S = N / R, NB = S * NSEQ;
+    constexpr int NG = (NB + 15) / 16, NW = NT / 64, GPW = (NG + NW - 1) / NW;
+    static_assert(NT % 64 == 0, "the matrix-core stage needs whole waves");
+    const int lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15, kb = lane >> 4;
+    const int r0 = (kb & 1) * 8;          // inputs r0 .. r0 + 7 of the butterfly
+    const bool im = kb >= 2;              // their imaginary parts (else real)
+    const Dft16Mfma A = dft16_operands<DIR>(lane);
+    bf16x8 bh[GPW], bl[GPW];
+    int jj[GPW], ss[GPW];
+#pragma unroll
+    for (int q = 0; q < GPW; q++) {
+        const int g = wave + q * NW, b = 16 * g + col;
+        int j, s;
+        if constexpr (ES == 1) { j = b % S; s = b / S; }
+        else { s = b % NSEQ; j = b / NSEQ; }
+        jj[q] = j;
+        ss[q] = s;
+        if (g < NG && b < NB) {
+            const int k = j % P;
+            const float2* rb = buf + lpos<ES, SS, PS>(j, s);
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int r = r0 + t;
+                float2 v = rb[lpos<ES, 0, PS>(r * S, 0)];
+                if constexpr (P > 1) {
+                    constexpr int STRIDE = N / (P * R);
+                    if (r > 0) {
+                        float2 w = tw[r * k * STRIDE];
+                        if (DIR > 0) w.y = -w.y;
+                        v = cmul(v, w);
+                    }
+                }
+                split_bf16(im ? v.y : v.x, bh[q], bl[q], t);
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 8; t++) bh[q][t] = bl[q][t] = (__bf16)0.f;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < GPW; q++) {
+        const int g = wave + q * NW;
+        if (g >= NG) continue;            // uniform over the wave
+        f32x4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
+        dre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.re_lo, bh[q], dre, 0, 0, 0);
+        dre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.re_hi, bl[q], dre, 0, 0, 0);
+        dre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.re_hi, bh[q], dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.im_lo, bh[q], dim, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.im_hi, bl[q], dim, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.im_hi, bh[q], dim, 0, 0, 0);
+        if (16 * g + col < NB) {
+            const int j = jj[q], s = ss[q], k = j % P;
+            float2* wb = buf + lpos<ES, SS, PS>((j - k) * R + k, s);
+#pragma unroll
+            for (int r = 0; r < 4; r++) wb[lpos<ES, 0, PS>((4 * kb + r) * P, 0)] = make_float2(dre[r], dim[r]);
+        }
+    }
+    __syncthreads();
+}
+
+// MF = 1: radix-16 stages on the matrix cores (float2 data only) where the
+// remaining length allows, the others as usual
+template <int N, int P, int NSEQ, int ES, int SS, int NT, int DIR, int PS, int MF = 0>
 struct fft_stages {
     template <typename C, typename TW>
     static __device__ __forceinline__ void run(C* buf, TW tw, int tid) {
-        constexpr int R = radix_of<N / P>::value;
-        fft_stage<N, P, R, NSEQ, ES, SS, NT, DIR, PS>(buf, tw, tid);
-        fft_stages<N, P * R, NSEQ, ES, SS, NT, DIR, PS>::run(buf, tw, tid);
+        if constexpr (MF && sizeof(C) == sizeof(float2) && (N / P) % 16 == 0) {
+            fft_stage_mfma16<N, P, NSEQ, ES, SS, NT, DIR, PS>(buf, tw, tid);
+            fft_stages<N, P * 16, NSEQ, ES, SS, NT, DIR, PS, MF>::run(buf, tw, tid);
+        } else {
+            constexpr int R = radix_of<N / P>::value;
+            fft_stage<N, P, R, NSEQ, ES, SS, NT, DIR, PS>(buf, tw, tid);
+            fft_stages<N, P * R, NSEQ, ES, SS, NT, DIR, PS, MF>::run(buf, tw, tid);
+        }
     }
 };
-template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS>
-struct fft_stages<N, N, NSEQ, ES, SS, NT, DIR, PS> {
+template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS, int MF>
+struct fft_stages<N, N, NSEQ, ES, SS, NT, DIR, PS, MF> {
     template <typename C, typename TW>
     static __device__ __forceinline__ void run(C*, TW, int) {}
 };
 
 // C = double2 (FP64 form) or float2 (mixed form), deduced from buf; tw is the
-// global twiddle table (const C*) or an LDS two-level table (TwL)
-template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS = 0, typename C, typename TW>
+// global twiddle table (const C*) or an LDS two-level table (TwL); MF = 1 puts
+// the radix-16 stages of a float2 transform on the matrix cores
+template <int N, int NSEQ, int ES, int SS, int NT, int DIR, int PS = 0, int MF = 0, typename C, typename TW>
 __device__ __forceinline__ void lds_fft(C* buf, TW tw, int tid) {
-    fft_stages<N, 1, NSEQ, ES, SS, NT, DIR, PS>::run(buf, tw, tid);
+    fft_stages<N, 1, NSEQ, ES, SS, NT, DIR, PS, MF>::run(buf, tw, tid);
 }
 
 // Twiddles w^m = exp(-2 pi i m / N) from two small LDS tables holding entries

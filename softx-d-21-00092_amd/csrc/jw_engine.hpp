@@ -73,7 +73,7 @@ __device__ __forceinline__ int xcd_strip(int b, int nstrips) {
 // row-major, 1 row-major for the 64-byte-strip column pass, 4 and 8 row blocks)
 template <int N, int BF>
 __device__ __forceinline__ size_t lay(int x, int y) {
-    if constexpr (BF <= 2) return (size_t)x * N + y;
+    if constexpr (BF <= 2 || BF == 16) return (size_t)x * N + y;
     else return ((size_t)(x / BF) * N + y) * BF + (x % BF);
 }
 
@@ -93,7 +93,7 @@ __device__ __forceinline__ size_t pat(int q, int x, int y, size_t V) {
 // consecutive turns (a bijection on [0, N) for N a multiple of 8 BF).
 template <int N, int BF>
 __device__ __forceinline__ int row_of_block(int b) {
-    if constexpr (BF <= 2) return b;
+    if constexpr (BF <= 2 || BF == 16) return b;
     else {
         static_assert(N % (8 * BF) == 0, "blocked layout needs N a multiple of 8 BF");
         const int t = b >> 3, xcd = b & 7;
@@ -204,14 +204,62 @@ __device__ __forceinline__ void row_noise_body(C* lds, C* __restrict__ Xi, size_
         for (int q = 0; q < SPT; q++) Xi[pat<N, BF>(4 + p, x, tid + q * NT, V)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
 }
 
+// The noise pass proper: one plane pair at a time.  Plane p of xi_x holds
+// (n_2p, n_2p+1) and plane p of xi_y (m_2p, m_2p+1), the normals of Philox
+// calls 2p and 2p+1 (draw order n0, m0, n1, m1, ...: call q gives n_q, m_q),
+// so each pair is generated, transformed and stored on its own and nothing is
+// held across pairs: two padded rows in LDS (70 KB in FP64 at N = 2048) fit
+// two workgroups per CU (FP32: 35 KB, four); the waves-per-SIMD bound asks
+// the compiler for the registers of that many.
+template <int N, typename C>
+struct NoiseWaves { static constexpr int value = sizeof(C) == sizeof(double2) ? 4 : 8; };
+
 template <int N, int BF, typename C>
-__global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_noise_fwd(C* __restrict__ Xi, size_t V, NoiseSrc ns,
+__global__ __launch_bounds__(RowE<N>::NT, (NoiseWaves<N, C>::value)) void k_row_noise_fwd(C* __restrict__ Xi, size_t V, NoiseSrc ns,
                                                               const double* __restrict__ sa_pos,
                                                               const C* __restrict__ tw) {
+    typedef typename scalar_of<C>::type T;
+    constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;
-    __shared__ C lds[4 * PN];
-    row_noise_body<N, BF, C>(lds, Xi + zofs(8, V), V, batch_noise(ns, V), sa_pos,
-                             tw_source<N, true>(tw), row_of_block<N, BF>(blockIdx.x));
+    __shared__ C lds[2 * PN];
+    const auto twl = tw_source<N, true>(tw);
+    Xi += zofs(8, V);
+    ns = batch_noise(ns, V);
+    const int x = row_of_block<N, BF>(blockIdx.x);
+    const size_t row0 = (size_t)x * N;
+    const int tid = threadIdx.x;
+#pragma unroll 1
+    for (int p = 0; p < 4; p++) {
+#pragma unroll
+        for (int q = 0; q < SPT; q++) {
+            const int y = site_of<C>(tid + q * NT);
+            const size_t i = row0 + y;
+            T z[4];
+            if (ns.inj) {
+                const double* zi = ns.inj + 16 * i + 4 * p;
+#pragma unroll
+                for (int k = 0; k < 4; k++) z[k] = (T)zi[k];
+            } else if constexpr (sizeof(T) == sizeof(double)) {
+                box_muller_pair((uint32_t)i, ns.step, ns.tag, (uint32_t)(2 * p), ns.seed, ns.traj, z[0], z[1]);
+                box_muller_pair((uint32_t)i, ns.step, ns.tag, (uint32_t)(2 * p + 1), ns.seed, ns.traj, z[2], z[3]);
+            } else {
+                box_muller_pair_f((uint32_t)i, ns.step, ns.tag, (uint32_t)(2 * p), ns.seed, ns.traj, z[0], z[1]);
+                box_muller_pair_f((uint32_t)i, ns.step, ns.tag, (uint32_t)(2 * p + 1), ns.seed, ns.traj, z[2], z[3]);
+            }
+            const T s = sa_pos ? (T)sa_pos[i] : (T)1;
+            lds[lpos<1, PN, 4>(y, 0)] = C{z[0] * s, z[2] * s};      // xi_x plane p: (n_2p, n_2p+1)
+            lds[lpos<1, PN, 4>(y, 1)] = C{z[1] * s, z[3] * s};      // xi_y plane p: (m_2p, m_2p+1)
+        }
+        __syncthreads();
+        lds_fft<N, 2, 1, PN, NT, -1, 4>(lds, tw_fft(twl), opaque_v(tid));
+#pragma unroll
+        for (int q = 0; q < SPT; q++) {
+            const int y = tid + q * NT;
+            Xi[pat<N, BF>(p, x, y, V)] = lds[lpos<1, PN, 4>(y, 0)];
+            Xi[pat<N, BF>(4 + p, x, y, V)] = lds[lpos<1, PN, 4>(y, 1)];
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -352,7 +400,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
 // LDS row x holds [X(x, c0..c0+W-1), Y(x, c0..c0+W-1)]: 2W sequences of length N
 // with element stride 2W.
 // ---------------------------------------------------------------------------
-template <int N, int CORR, typename C, int PI = 0>
+template <int N, int CORR, typename C, int PI = 0, int MF = 0>
 __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
                                                           C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                           const double* __restrict__ q4,
@@ -394,7 +442,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
         lds[lpos<S2, 1, 3>(x, s)] = (PI || s < W) ? X[g] : Y[g];
     }
     __syncthreads();
-    lds_fft<N, S2, S2, 1, NT, -1, 3>(lds, tw_fft(twl), opaque_v(tid));
+    lds_fft<N, S2, S2, 1, NT, -1, 3, MF>(lds, tw_fft(twl), opaque_v(tid));
     // pair (x, c): X^ in slot c of LDS row x, Y^ in slot W + c
     C zreg[CORR ? PAIRS : 1];
     // FP32 correlated: indices recomputed per phase (opaque_v), since addresses
@@ -429,7 +477,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
     __syncthreads();
     if (CORR) {
         // xi~ = F_col^-1 of the scaled tiles, written back over X and Y
-        lds_fft<N, S2, S2, 1, NT, +1, 3>(lds, tw_fft(twl), opaque_v(tid));
+        lds_fft<N, S2, S2, 1, NT, +1, 3, MF>(lds, tw_fft(twl), opaque_v(tid));
         const int ts = REIDX ? opaque_v(tid) : tid;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
@@ -449,7 +497,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
         __syncthreads();
     }
     // Z: the W sequences in the X slots
-    lds_fft<N, W, S2, 1, NT, +1, 3>(lds, tw_fft(twl), opaque_v(tid));
+    lds_fft<N, W, S2, 1, NT, +1, 3, MF>(lds, tw_fft(twl), opaque_v(tid));
     const int to = REIDX ? opaque_v(tid) : tid;
 #pragma unroll
     for (int q = 0; q < PAIRS; q++) {
@@ -472,27 +520,31 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restr
 template <int N, int BF, typename C>
 struct ColB {
     // BF = 4, 8: one 128-byte segment (SEG elements) holds CPB columns of BF rows;
-    // BF = 1 (row-major): 64-byte row segments of CPB adjacent columns
-    static constexpr int SEG = BF == 1 ? 64 / (int)sizeof(C) : 128 / (int)sizeof(C);
-    static constexpr int CPB = SEG / BF;               // columns per workgroup
+    // BF = 1 (row-major): 64-byte row segments of CPB adjacent columns;
+    // BF = 16 (row-major): 32-byte row segments (two workgroups per CU in FP64)
+    static constexpr int RB = (BF == 1 || BF == 16) ? 1 : BF;     // rows per block of the layout
+    static constexpr int SEG = BF == 1 ? 64 / (int)sizeof(C) : BF == 16 ? 32 / (int)sizeof(C) : 128 / (int)sizeof(C);
+    static constexpr int CPB = SEG / RB;               // columns per workgroup
     static constexpr int ELEMS = CPB * N;
     static constexpr int NTMAX = BF == 1 ? 1024 : 512;
     static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > NTMAX) ? NTMAX : ELEMS / 8);
     static constexpr int PER = ELEMS / NT;             // elements per thread
-    static_assert((BF == 1 || BF == 4 || BF == 8) && CPB >= 1 && CPB <= N && PER * NT == ELEMS && N % 16 == 0 &&
-                      N % CPB == 0,
+    // BF = 16 asks for the registers of two workgroups per CU (its LDS allows two)
+    static constexpr int WAVES = BF == 16 ? (NT * 2 + 255) / 256 : 1;
+    static_assert((BF == 1 || BF == 4 || BF == 8 || BF == 16) && CPB >= 1 && CPB <= N && PER * NT == ELEMS &&
+                      N % 16 == 0 && N % CPB == 0,
                   "blocked column tiling");
 };
 
 template <int N, int BF, int CORR, typename C>
-__global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __restrict__ Z, C* __restrict__ X,
+__global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void k_col_kmult_blk(C* __restrict__ Z, C* __restrict__ X,
                                                                      C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                                      const double* __restrict__ q4,
                                                                      const double* __restrict__ sb,
                                                                      const double* __restrict__ sa16,
                                                                      const C* __restrict__ tw) {
     typedef typename scalar_of<C>::type T;
-    constexpr int SEG = ColB<N, BF, C>::SEG, CPB = ColB<N, BF, C>::CPB;
+    constexpr int SEG = ColB<N, BF, C>::SEG, CPB = ColB<N, BF, C>::CPB, RB = ColB<N, BF, C>::RB;
     constexpr int NT = ColB<N, BF, C>::NT, PER = ColB<N, BF, C>::PER;
     constexpr int PN = padded<N, 4>::value;
     __shared__ C lds[CPB * PN];
@@ -501,15 +553,16 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT)) void k_col_kmult_blk(C* __res
     Z += zofs(4, V);
     X += zofs(8, V);
     Y += zofs(8, V);
-    const int y0 = blockIdx.x * CPB;
+    // 32-byte strips: the four strips of one 128-byte line on one XCD at neighbouring times
+    const int y0 = (BF == 16 ? xcd_strip(blockIdx.x, N / CPB) : (int)blockIdx.x) * CPB;
     const double invV = 1.0 / ((double)N * (double)N);
     // global offset within a plane / LDS slot of loaded element e: x-block
     // e / SEG, slot e mod SEG = c BF + x mod BF (row-major, BF = 1: row e / SEG,
     // column y0 + e mod SEG)
-    auto goff = [&](int e) -> size_t { return (size_t)(e / SEG) * ((size_t)N * BF) + (size_t)y0 * BF + (e % SEG); };
+    auto goff = [&](int e) -> size_t { return (size_t)(e / SEG) * ((size_t)N * RB) + (size_t)y0 * RB + (e % SEG); };
     auto lofs = [&](int e) -> int {
         const int sl = e % SEG;
-        return lpos<1, PN, 4>((e / SEG) * BF + sl % BF, sl / BF);
+        return lpos<1, PN, 4>((e / SEG) * RB + sl % RB, sl / RB);
     };
     auto load = [&](const C* __restrict__ P) {
         const int t = opaque_v(tid);
@@ -705,6 +758,18 @@ static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int 
     const C* tw = twiddles<C>(c);
     if constexpr (BF == 0 || BF == 2) {
         constexpr int W = ColE<N, C>::W, PI = BF == 2;
+        // split-BF16 matrix-core radix-16 stages: FP32 storage, row-major layout
+        if constexpr (sizeof(C) == sizeof(float2) && BF == 0) {
+            if (c->fft_mfma) {
+                if (corr)
+                    hipLaunchKernelGGL((k_col_kmult<N, 1, C, 0, 1>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 1>::NT), 0,
+                                       c->stream, Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
+                else
+                    hipLaunchKernelGGL((k_col_kmult<N, 0, C, 0, 1>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 0>::NT), 0,
+                                       c->stream, Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
+                return hipGetLastError();
+            }
+        }
         if (corr)
             hipLaunchKernelGGL((k_col_kmult<N, 1, C, PI>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 1>::NT), 0, c->stream,
                                Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
@@ -779,6 +844,7 @@ static hipError_t step_fused_layout(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB,
         case 2: return step_fused_n<N, 2>(c, U, Xi, Vv, ZA, ZB, a, kcls);
         case 4: return step_fused_n<N, 4>(c, U, Xi, Vv, ZA, ZB, a, kcls);
         case 8: return step_fused_n<N, 8>(c, U, Xi, Vv, ZA, ZB, a, kcls);
+        case 16: return step_fused_n<N, 16>(c, U, Xi, Vv, ZA, ZB, a, kcls);
         default: return hipErrorInvalidValue;
     }
 }

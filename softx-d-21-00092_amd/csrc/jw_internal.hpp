@@ -73,6 +73,12 @@ struct jw_ctx {
     // 32-byte column strips in LDS), 1 row-major (one plane at a time over
     // 64-byte strips), 4 or 8 rows per block, -1 default
     int layout;
+    // 1: the FP32 row-major column passes run their radix-16 stages as
+    // split-BF16 DFT-matrix products on the matrix cores (jw_set_mixed_fft)
+    int fft_mfma;
+    // 1: solve_poisson sets the k = 0 mode to zero instead of multiplying it
+    // by -g/mass^2 (jw_set_mv_zero_mode)
+    int mv_zero_mode;
     // per-kernel timing (HIP events on ctx->stream)
     int timing;
     double kernel_ms[16];
@@ -86,7 +92,8 @@ hipError_t fft2d(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale);
 hipError_t fft_rows(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale);
 hipError_t fft_cols(jw_ctx* c, cplx* planes, int nplanes, int dir, double scale);
 
-hipError_t build_sqrt_alpha(jw_ctx* c, int variant, const Alpha& a);
+// sqrt(alpha) over the lattice into sa (V doubles)
+hipError_t build_sqrt_alpha(jw_ctx* c, double* sa, int variant, const Alpha& a);
 
 // reference per-function kernels on 9-component (SoA) fields
 hipError_t set_unit(jw_ctx* c, cplx* U);
@@ -102,6 +109,10 @@ hipError_t trace(jw_ctx* c, const cplx* U, cplx* corr);
 hipError_t average_and_symmetrize(jw_ctx* c, const cplx* in, cplx* out);
 hipError_t reduce(jw_ctx* c, const cplx* corr, cplx* sum, cplx* err);
 hipError_t scale_planes(jw_ctx* c, cplx* p, size_t n, double s);
+hipError_t scale_by_table(jw_ctx* c, cplx* f, int t, const double* s);
+// POSITION_EVOLUTION kernel sums over all site pairs (weights per displacement)
+hipError_t position_sums(jw_ctx* c, cplx* A, cplx* B, const cplx* X, const cplx* Y, const cplx* VX, const cplx* VY,
+                         const double* wa, const double* wb);
 
 // generator-basis engine: the unfused step (kc: 0 for SQRT coupling, 1 otherwise)
 hipError_t step_unfused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const NoiseSrc& ns,

@@ -50,8 +50,8 @@ __global__ __launch_bounds__(256) void k_sqrt_alpha(double* __restrict__ sa, con
     }
 }
 
-hipError_t build_sqrt_alpha(jw_ctx* c, int variant, const Alpha& a) {
-    hipLaunchKernelGGL(k_sqrt_alpha, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, c->sa, c->q4, c->N,
+hipError_t build_sqrt_alpha(jw_ctx* c, double* sa, int variant, const Alpha& a) {
+    hipLaunchKernelGGL(k_sqrt_alpha, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, sa, c->q4, c->N,
                        variant, a);
     return hipGetLastError();
 }
@@ -214,9 +214,11 @@ hipError_t set_mv_model(jw_ctx* c, cplx* f, const NoiseSrc& ns, double sigma) {
     return hipGetLastError();
 }
 
-// solvePoisson (field.h:867-877): f(k) *= -g/(-phat2(k) + mass^2)
+// solvePoisson (field.h:867-877): f(k) *= -g/(-phat2(k) + mass^2).  The
+// reference multiplies the k = 0 mode by -g/mass^2 (no guard); with zero_mode
+// set it is set to zero instead (the option of projecting out the zero mode).
 __global__ __launch_bounds__(256) void k_solve_poisson(cplx* __restrict__ f, int t, int N, const double* __restrict__ q4, double mass,
-                                double g) {
+                                double g, int zero_mode) {
 #pragma clang fp contract(off)
     const size_t V = (size_t)N * N;
     GRID_STRIDE(i, V) {
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(256) void k_solve_poisson(cplx* __restrict__ f, int
         const double fac = -1.0 * g / (-ph + mass * mass);
         for (int k = 0; k < t; k++) {
             const cplx v = f[k * V + i];
-            f[k * V + i] = xmul(v, mk(fac, 0.0));
+            f[k * V + i] = (zero_mode && i == 0) ? mk(0.0, 0.0) : xmul(v, mk(fac, 0.0));
         }
     }
 }
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(256) void k_solve_poisson(cplx* __restrict__ f, int
 hipError_t solve_poisson(jw_ctx* c, cplx* f, int t, double mass, double g) {
     KTimer t_(c, K_DROPIN);
     hipLaunchKernelGGL(k_solve_poisson, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, f, t, c->N, c->q4,
-                       mass, g);
+                       mass, g, c->mv_zero_mode);
     return hipGetLastError();
 }
 
@@ -354,6 +356,88 @@ __global__ __launch_bounds__(256) void k_scale(cplx* __restrict__ p, size_t n, d
 
 hipError_t scale_planes(jw_ctx* c, cplx* p, size_t n, double s) {
     hipLaunchKernelGGL(k_scale, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, p, n, s);
+    return hipGetLastError();
+}
+
+// ===========================================================================
+// POSITION_EVOLUTION (main_optimized.cpp:271-297, field.h:2368-2519): the
+// position-space kernel sums at every site by brute force, O(V^2):
+//   A(x) = sum_y wa(x - y) xi_x(y) + wb(x - y) xi_y(y)
+//   B(x) = sum_y wa(x - y) V_x(y) + wb(x - y) V_y(y),   V_i = U xi_i U^dagger
+// (the reference forms U E U^dagger per pair; with V_i precomputed the sum is
+// the same to rounding).  One thread per target site; the sources pass through
+// LDS in tiles of JW_POS_TILE (xi_x, xi_y, V_x, V_y: 36 complex values each).
+// Each term is rounded as the reference rounds C = a xi_x, D = b xi_y, E = C + D.
+// ===========================================================================
+#define JW_POS_TILE 32
+
+__global__ __launch_bounds__(256) void k_position_sums(cplx* __restrict__ A, cplx* __restrict__ B,
+                                                       const cplx* __restrict__ X, const cplx* __restrict__ Y,
+                                                       const cplx* __restrict__ VX, const cplx* __restrict__ VY,
+                                                       const double* __restrict__ wa, const double* __restrict__ wb,
+                                                       int N) {
+#pragma clang fp contract(off)
+    const size_t V = (size_t)N * N;
+    __shared__ cplx tile[4][9][JW_POS_TILE];
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = t < V;
+    const int tx = (int)(t / N), ty = (int)(t % N);
+    cplx a[9], b[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) a[k] = b[k] = mk(0.0, 0.0);
+    for (size_t y0 = 0; y0 < V; y0 += JW_POS_TILE) {
+        for (int e = threadIdx.x; e < 4 * 9 * JW_POS_TILE; e += blockDim.x) {
+            const int f = e / (9 * JW_POS_TILE), k = (e / JW_POS_TILE) % 9, j = e % JW_POS_TILE;
+            const size_t src = y0 + j;
+            const cplx* P = f == 0 ? X : f == 1 ? Y : f == 2 ? VX : VY;
+            tile[f][k][j] = src < V ? P[k * V + src] : mk(0.0, 0.0);
+        }
+        __syncthreads();
+        const int nj = (int)(V - y0 < JW_POS_TILE ? V - y0 : JW_POS_TILE);
+        if (active)
+            for (int j = 0; j < nj; j++) {
+                const size_t src = y0 + j;
+                const int sx = (int)(src / N), sy = (int)(src % N);
+                const size_t r = (size_t)((tx - sx + N) % N) * N + (size_t)((ty - sy + N) % N);
+                const double wx = wa[r], wy = wb[r];
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    const cplx e1 = xadd(xmul(mk(wx, 0.0), tile[0][k][j]), xmul(mk(wy, 0.0), tile[1][k][j]));
+                    const cplx e2 = xadd(xmul(mk(wx, 0.0), tile[2][k][j]), xmul(mk(wy, 0.0), tile[3][k][j]));
+                    a[k] = xadd(a[k], e1);
+                    b[k] = xadd(b[k], e2);
+                }
+            }
+        __syncthreads();
+    }
+    if (active)
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            A[k * V + t] = a[k];
+            B[k * V + t] = b[k];
+        }
+}
+
+hipError_t position_sums(jw_ctx* c, cplx* A, cplx* B, const cplx* X, const cplx* Y, const cplx* VX, const cplx* VY,
+                         const double* wa, const double* wb) {
+    KTimer t(c, K_DROPIN);
+    hipLaunchKernelGGL(k_position_sums, dim3((unsigned)((c->V + 255) / 256)), dim3(256), 0, c->stream, A, B, X, Y, VX,
+                       VY, wa, wb, c->N);
+    return hipGetLastError();
+}
+
+// f(k) *= s(k) on t planes (the per-mode noise correlation)
+__global__ __launch_bounds__(256) void k_scale_by_table(cplx* __restrict__ f, int t, const double* __restrict__ s,
+                                                        size_t V) {
+    GRID_STRIDE(i, V) {
+        const double v = s[i];
+        for (int k = 0; k < t; k++) f[k * V + i] = cscale(f[k * V + i], v);
+    }
+}
+
+hipError_t scale_by_table(jw_ctx* c, cplx* f, int t, const double* s) {
+    KTimer t_(c, K_DROPIN);
+    hipLaunchKernelGGL(k_scale_by_table, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, f, t, s, c->V);
     return hipGetLastError();
 }
 

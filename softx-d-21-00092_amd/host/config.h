@@ -14,6 +14,17 @@
 //                           batched (all stat samples evolve together)
 //   precision = fp64        fp64 | mixed (FP32 fields and FFTs, FP64 exponentials)
 //   reunit_every = 10       mixed precision: steps between projections onto SU(3)
+//   float_parse = 1         1: mu, mass and step pass through a float as in the
+//                           reference; 0: they are read as doubles
+//   mv_zero_mode = 0        0: the reference's k = 0 factor -g/m^2 in solvePoisson;
+//                           1: each MV sheet's k = 0 mode is set to zero
+//   batch = 256:48,512:32   batched engine: size groups N:count (uneven sizes);
+//                           each group writes its own output file
+//   checkpoint_every = 0    steps between checkpoints of (U, noise state) per
+//                           stat sample (0: none); files <file_name>_ckpt_<stat>
+//   resume = 0              1: continue from the checkpoint files
+//   stop_after = 0          end the run after this many Langevin steps (0: never),
+//                           leaving the checkpoint for resume
 #ifndef JW_HOST_CONFIG_H
 #define JW_HOST_CONFIG_H
 
@@ -22,6 +33,8 @@
 #include <string.h>
 
 #include <string>
+#include <utility>
+#include <vector>
 
 enum Evolution { POSITION_EVOLUTION, MOMENTUM_EVOLUTION };
 
@@ -66,6 +79,12 @@ class config {
     Engine engine = ENGINE_FUSED;
     bool mixed = false;
     int reunit_every = 10;
+    bool float_parse = true;
+    int mv_zero_mode = 0;
+    std::vector<std::pair<int, int>> batch;     // (N, count) size groups
+    int checkpoint_every = 0;
+    bool resume = false;
+    int stop_after = 0;
 
     // Returns 1 on success, 0 when the file cannot be read or a required key is
     // missing (the reference returns nothing and reads garbage).
@@ -75,21 +94,18 @@ class config {
             fprintf(stderr, "config: cannot open %s\n", path.c_str());
             return 0;
         }
-        float tmp;
-        char buf[256];
+        // mu, mass and step are kept as text until float_parse is known
+        char buf[256], tmu[128], tmass[128], tstep[128];
         int ok = 1;
         ok &= fscanf(f, " stat = %i", &stat) == 1;
-        ok &= fscanf(f, " mu = %f", &tmp) == 1;
-        mu = tmp;
-        ok &= fscanf(f, " mass = %f", &tmp) == 1;
-        mass = tmp;
+        ok &= fscanf(f, " mu = %127s", tmu) == 1;
+        ok &= fscanf(f, " mass = %127s", tmass) == 1;
         ok &= fscanf(f, " elementaryWilsonLines = %i", &elementaryWilsonLines) == 1;
         ok &= fscanf(f, " file_name = %255s", buf) == 1;
         file_name = buf;
         ok &= fscanf(f, " langevin_steps = %i", &langevin_steps) == 1;
         ok &= fscanf(f, " measurements = %i", &measurements) == 1;
-        ok &= fscanf(f, " step = %f", &tmp) == 1;
-        step = tmp;
+        ok &= fscanf(f, " step = %127s", tstep) == 1;
         char evolution[100], coupling[100], kernel[100];
         ok &= fscanf(f, " EvolutionChoice = %99s", evolution) == 1;
         ok &= fscanf(f, " CouplingChoice = %99s", coupling) == 1;
@@ -131,14 +147,60 @@ class config {
                     ok = 0;
                 }
             } else if (!strcmp(key, "reunit_every")) reunit_every = atoi(val);
+            else if (!strcmp(key, "float_parse")) float_parse = atoi(val) != 0;
+            else if (!strcmp(key, "mv_zero_mode")) mv_zero_mode = atoi(val);
+            else if (!strcmp(key, "checkpoint_every")) checkpoint_every = atoi(val);
+            else if (!strcmp(key, "resume")) resume = atoi(val) != 0;
+            else if (!strcmp(key, "stop_after")) stop_after = atoi(val);
+            else if (!strcmp(key, "batch")) ok &= parse_batch(val);
             else fprintf(stderr, "config: unknown key %s ignored\n", key);
         }
         fclose(f);
+        // the reference's fscanf("%f") into a float (config.h:79-93), or doubles
+        ok &= to_real(tmu, &mu) && to_real(tmass, &mass) && to_real(tstep, &step);
+        if (!batch.empty()) engine = ENGINE_BATCHED;
+        if (checkpoint_every < 0 || stop_after < 0 || (mv_zero_mode != 0 && mv_zero_mode != 1)) {
+            fprintf(stderr, "config: checkpoint_every and stop_after must be >= 0, mv_zero_mode 0 or 1\n");
+            ok = 0;
+        }
         if (engine == ENGINE_REFERENCE && mixed) {
             fprintf(stderr, "config: the reference engine runs in FP64 only\n");
             ok = 0;
         }
         return ok;
+    }
+
+   private:
+    int to_real(const char* t, double* out) const {
+        char* end = nullptr;
+        if (float_parse) *out = (double)strtof(t, &end);
+        else *out = strtod(t, &end);
+        if (end == t || *end) {
+            fprintf(stderr, "config: %s is not a number\n", t);
+            return 0;
+        }
+        return 1;
+    }
+
+    // "N:count,N:count,..."
+    int parse_batch(const char* v) {
+        batch.clear();
+        const char* p = v;
+        while (*p) {
+            int n = 0, cnt = 0, used = 0;
+            if (sscanf(p, "%d:%d%n", &n, &cnt, &used) != 2 || n < 1 || cnt < 1) {
+                fprintf(stderr, "config: batch %s: expected N:count[,N:count...]\n", v);
+                return 0;
+            }
+            batch.emplace_back(n, cnt);
+            p += used;
+            if (*p == ',') p++;
+            else if (*p) {
+                fprintf(stderr, "config: batch %s: expected N:count[,N:count...]\n", v);
+                return 0;
+            }
+        }
+        return !batch.empty();
     }
 };
 

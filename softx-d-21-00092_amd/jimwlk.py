@@ -8,6 +8,7 @@ no CPU fallback: loading fails when the HIP library is missing.
 import ctypes
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -65,6 +66,9 @@ def lib():
         L.jw_field_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.jw_noise_inject.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
         L.jw_noise_philox.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.jw_noise_get_step.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
+        L.jw_checkpoint_write.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p]
+        L.jw_checkpoint_read.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p]
         L.jw_evolve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 ctypes.POINTER(Alpha), ctypes.c_int]
         L.jw_evolve_mixed.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -74,6 +78,9 @@ def lib():
         L.jw_update_uf.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_double]
         L.jw_set_mv_model.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.jw_solve_poisson.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.jw_set_mv_zero_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_prepare_A_and_B_local.argtypes = [ctypes.c_void_p] * 6 + [ctypes.c_int, ctypes.c_int]
+        L.jw_correlate_noise.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Alpha)]
         L.jw_prepare_A_local.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Alpha)]
         L.jw_generate_gaussian_with_noise_coupling_constant.argtypes = [ctypes.c_void_p] * 3 + [ctypes.POINTER(Alpha)]
         L.jw_measure_corr.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp]
@@ -84,6 +91,7 @@ def lib():
         L.jw_su3_exp_gen.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp, ctypes.c_int]
         L.jw_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_set_fused.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_set_mixed_fft.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_calibration_copy.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -117,6 +125,7 @@ class Field:
         h = ctypes.c_void_p()
         _check(lib().jw_field_create(ctx.h, t, ctypes.byref(h)))
         self.h = h
+        ctx._fields.add(self)
 
     def upload(self, a):
         a = np.ascontiguousarray(a, dtype=np.complex128).reshape(self.ctx.V, self.t)
@@ -147,9 +156,14 @@ class Context:
         h = ctypes.c_void_p()
         _check(lib().jw_create(ctypes.byref(h), N, device))
         self.h, self.N, self.V = h, N, N * N
+        self._fields = weakref.WeakSet()
 
     def close(self):
+        """Destroys the context after its remaining fields (a field must not outlive
+        its context: jw_field_destroy uses the context's stream)."""
         if self.h:
+            for f in list(self._fields):
+                f.close()
             lib().jw_destroy(self.h)
             self.h = None
 
@@ -181,6 +195,19 @@ class Context:
     def philox(self, seed, traj=0, step=0):
         _check(lib().jw_noise_philox(self.h, seed, traj, step))
 
+    def noise_step(self):
+        s = ctypes.c_uint32()
+        _check(lib().jw_noise_get_step(self.h, ctypes.byref(s)))
+        return s.value
+
+    def checkpoint_write(self, U, path):
+        """jw_checkpoint_write: U and the Philox state to a file."""
+        _check(lib().jw_checkpoint_write(self.h, U.h, os.fsencode(path)))
+
+    def checkpoint_read(self, U, path):
+        """jw_checkpoint_read: U and the Philox state from a file."""
+        _check(lib().jw_checkpoint_read(self.h, U.h, os.fsencode(path)))
+
     # reference per-function entry points
     def execute2D(self, f, dir):
         _check(lib().jw_execute2D(self.h, f.h, int(dir)))
@@ -201,11 +228,23 @@ class Context:
     def update_uf(self, U, B, A, step):
         _check(lib().jw_update_uf(self.h, U.h, B.h, A.h, step))
 
+    def prepare_A_and_B_local(self, A, B, xx, xy, U, coupling, kernel):
+        """POSITION_EVOLUTION kernel sums at every site (O(V^2), N <= 256)."""
+        _check(lib().jw_prepare_A_and_B_local(self.h, A.h, B.h, xx.h, xy.h, U.h, coupling, kernel))
+
+    def correlate_noise(self, f, al=None):
+        """f <- F^-1[sqrt(alpha(k)) F f] (the noise correlation of the position path)."""
+        _check(lib().jw_correlate_noise(self.h, f.h, ctypes.byref(al) if al else None))
+
     def set_mv_model(self, f, g, mu, Ny):
         _check(lib().jw_set_mv_model(self.h, f.h, g, mu, Ny))
 
     def solve_poisson(self, f, mass, g):
         _check(lib().jw_solve_poisson(self.h, f.h, mass, g))
+
+    def set_mv_zero_mode(self, mode):
+        """0: the reference's k = 0 factor -g/m^2 in solvePoisson; 1: the k = 0 mode set to zero."""
+        _check(lib().jw_set_mv_zero_mode(self.h, int(mode)))
 
     def mul_exp(self, U, f):
         _check(lib().jw_mul_exp(self.h, U.h, f.h))
@@ -258,14 +297,21 @@ class Context:
         _check(lib().jw_calibration_copy(self.h, nplanes))
 
     # jw_set_fused modes
-    UNFUSED, FUSED, FUSED_ROWMAJOR, FUSED_BLOCK4, FUSED_BLOCK8, FUSED_ROWMAJOR_WIDE, FUSED_INTERLEAVED = range(7)
+    (UNFUSED, FUSED, FUSED_ROWMAJOR, FUSED_BLOCK4, FUSED_BLOCK8, FUSED_ROWMAJOR_WIDE, FUSED_INTERLEAVED,
+     FUSED_ROWMAJOR_NARROW) = range(8)
 
     def set_fused(self, mode=1):
         """mode: False/0 unfused kernel sequence, True/1 fused (default layout),
         2/3/4 fused with row-major / 4-row / 8-row blocked intermediate planes,
         5 row-major with the column pass over one plane of 64-byte strips, 6 the
-        two directions of each plane pair interleaved site by site."""
+        two directions of each plane pair interleaved site by site, 7 row-major with
+        the column pass over one plane of 32-byte strips."""
         _check(lib().jw_set_fused(self.h, int(mode)))
+
+    def set_mixed_fft(self, mode):
+        """0: FP32 butterflies; 1: radix-16 stages of the FP32 column passes as split-BF16
+        DFT-matrix products on the matrix cores."""
+        _check(lib().jw_set_mixed_fft(self.h, int(mode)))
 
     def kernel_times(self):
         out = {}

@@ -196,3 +196,63 @@ def test_mixed_rejects_unsupported(jw):
     with pytest.raises(jw.JwError):
         c.evolve_mixed(U, 2, 1, 1e-4, None, 1)     # HATTA
     c.close()
+
+
+# ------------------------------------------------------------ split-BF16 matrix-core stages
+@pytest.mark.parametrize("N,coupling", [(64, 3), (256, 4), (2048, 4)])
+def test_mixed_mfma_fft_one_step(jw, N, coupling):
+    """The column transforms' radix-16 stages on the matrix cores (split BF16, three
+    products, FP32 accumulation; jw_set_mixed_fft(1)): one step from the same U and
+    Philox stream against FP64.  Each such stage carries about 2^-17 relative error,
+    so the bound is 2e-5 per site (the FP32 butterflies: 1e-6); the FP32-butterfly
+    result must lie closer to FP64 than this one does."""
+    c = jw.Context(N)
+    U0 = mv_start(jw, c, Ny=2)
+    al = alpha_for(jw, N, coupling)
+    ds = 5e-5 if coupling == 4 else 1e-4
+    res = {}
+    for name in ("fp64", "fp32", "mfma"):
+        F = c.field(9, U0)
+        c.philox(17, 3, 40)
+        c.set_mixed_fft(1 if name == "mfma" else 0)
+        if name == "fp64":
+            c.evolve(F, coupling, 1, ds, al, nsteps=1)
+        else:
+            c.evolve_mixed(F, coupling, 1, ds, al, nsteps=1, reunit_every=0)
+        res[name] = F.download()
+        F.close()
+    c.set_mixed_fft(0)
+    e_mfma, e_fp32 = frob_max(res["fp64"], res["mfma"]), frob_max(res["fp64"], res["fp32"])
+    print(f"mfma one step N={N}: |mfma - fp64| {e_mfma:.2e}, |fp32 - fp64| {e_fp32:.2e}", flush=True)
+    assert e_mfma <= 2e-5, e_mfma
+    assert e_fp32 < e_mfma          # the matrix-core stages did run
+    assert frob_max(res["fp64"], U0) > 1e-3
+    c.close()
+
+
+@pytest.mark.parametrize("N", [256, 2048])
+def test_mixed_mfma_fft_dipole_1000_steps(jw, N):
+    """1000 correlated steps (alpha_max = 0.7, Lambda a = 0.02, ds = 5e-5) of the
+    mixed form with matrix-core column stages from the same MV start and Philox
+    stream as FP64: S(r) within 2e-5 absolute."""
+    c = jw.Context(N)
+    U0 = mv_start(jw, c, Ny=10)
+    al = alpha_for(jw, N, 4)
+    S = {}
+    for name in ("fp64", "mfma"):
+        F = c.field(9, U0)
+        c.philox(21, 4, 0)
+        if name == "fp64":
+            c.evolve(F, 4, 1, 5e-5, al, nsteps=1000)
+        else:
+            c.set_mixed_fft(1)
+            c.evolve_mixed(F, 4, 1, 5e-5, al, nsteps=1000, reunit_every=10)
+            c.set_mixed_fft(0)
+        S[name] = c.dipole_from_corr(c.measure_corr(F))
+        F.close()
+    S0 = c.dipole_from_corr(c.measure_corr(c.field(9, U0)))
+    err = np.abs(S["fp64"] - S["mfma"]).max()
+    print(f"mfma 1000 steps N={N}: max |S_mfma - S_fp64| = {err:.2e}", flush=True)
+    assert err <= 2e-5, err
+    assert np.abs(S["fp64"] - S0).max() > 1e-3
+    c.close()

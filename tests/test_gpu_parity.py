@@ -11,6 +11,8 @@ Tolerances (per-site max-abs over matrix entries unless stated):
   * per-site Frobenius norm of U_gpu - U_ref <= 1e-12 after one Langevin step
     and <= 1e-9 after 1000 steps (N = 64, injected identical noise).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -359,6 +361,104 @@ def test_mv_init_against_oracle(jw, orc, ctx64):
     ctx64.mv_init(U, Ny, g, mu, 1e-4 / (g * g * mu), z)
     ref = orc.mv_init(N, z, Ny, g, mu, 1e-4 / (g * g * mu))
     assert frob_max(U.download(), ref) <= 1e-7
+
+
+@pytest.mark.parametrize("N,tol", [(64, 1e-12), (256, 3e-12)])
+def test_mv_init_zero_mode_projected_against_oracle(jw, orc, N, tol):
+    """m a = 1e-4 (Ny = 50, g^2 mu a = 0.25) with the zero mode projected out of each
+    sheet: the start matches the oracle to 1e-12 per site at N = 64.  The remaining
+    infrared modes are amplified by 1/phat2 ~ (N / 2 pi)^2, so the rounding of two
+    correct FP64 paths grows with N: the oracle and a numpy-FFT composition of its
+    per-site functions differ by 1.3e-13 at N = 64 and 7.5e-13 at N = 256 (50 sheets),
+    hence 3e-12 there."""
+    Ny, g, mu = 50, 1.0, 0.25
+    c = jw.Context(N)
+    c.set_mv_zero_mode(1)
+    z = rng.normal(size=Ny * N * N * 8)
+    U = c.field()
+    c.mv_init(U, Ny, g, mu, 1e-4 / (g * g * mu), z)
+    ref = orc.mv_init(N, z, Ny, g, mu, 1e-4 / (g * g * mu), zero_mode=1)
+    got = U.download()
+    assert frob_max(got, ref) <= tol
+    # the per-function path honours the option too: one sheet through
+    # set_mv_model -> execute2D -> solvePoisson -> execute2D -> operator*=
+    c.inject(z[: 8 * N * N], per_site=8)
+    f, W = c.field(), c.field()
+    c.set_unit(W)
+    c.set_mv_model(f, g, mu, Ny)
+    c.execute2D(f, 1)
+    c.solve_poisson(f, 1e-4 / (g * g * mu) * g * g * mu, g)
+    c.execute2D(f, 0)
+    c.mul_exp(W, f)
+    ref1 = orc.mv_init(N, z[: 8 * N * N], 1, g, mu / math.sqrt(Ny), 1e-4 / (g * g * mu) * math.sqrt(Ny),
+                       zero_mode=1)
+    assert frob_max(W.download(), ref1) <= 1e-12
+    c.set_mv_zero_mode(0)
+    c.mv_init(U, Ny, g, mu, 1e-4 / (g * g * mu), z)
+    assert frob_max(U.download(), got) > 1e-3      # the option changed the start
+    c.close()
+
+
+# ------------------------------------------------------------------ POSITION_EVOLUTION
+@pytest.mark.parametrize("N", [16, 64])
+@pytest.mark.parametrize("coupling", [0, 1, 3])
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_prepare_A_and_B_local_against_oracle(jw, orc, N, coupling, kernel):
+    """The O(V^2) position-space sums (field.h:2368-2519 at every site) against the
+    oracle's restatement (pinned to the reference build); the summation orders
+    differ, hence the relative bound."""
+    c = jw.Context(N)
+    U0 = random_su3(orc, N * N)
+    xx, xy = orc.generate_gaussian(N, rng.normal(size=N * N * 16))
+    A, B = c.field(), c.field()
+    c.prepare_A_and_B_local(A, B, c.field(9, xx), c.field(9, xy), c.field(9, U0), coupling, kernel)
+    Ao, Bo = orc.prepare_A_and_B_local(xx, xy, U0, N, coupling, kernel)
+    assert np.abs(A.download() - Ao).max() <= 1e-13 * np.abs(Ao).max()
+    assert np.abs(B.download() - Bo).max() <= 1e-13 * np.abs(Bo).max()
+    c.close()
+
+
+def test_position_evolution_steps_against_oracle(jw, orc):
+    """Two POSITION_EVOLUTION steps (main_optimized.cpp:243-300) through the
+    per-function calls on injected noise, NOISE coupling: white noise, per-mode
+    correlation (jw_correlate_noise), the position-space sums, update_uf."""
+    N, ds = 32, 1e-3
+    c = jw.Context(N)
+    U0 = random_su3(orc, N * N)
+    U = c.field(9, U0)
+    xx, xy, A, B = c.field(), c.field(), c.field(), c.field()
+    Uo = U0
+    for _ in range(2):
+        z = rng.normal(size=N * N * 16)
+        c.inject(z)
+        c.generate_gaussian(xx, xy)
+        c.correlate_noise(xx)
+        c.correlate_noise(xy)
+        c.prepare_A_and_B_local(A, B, xx, xy, U, 1, 1)
+        c.update_uf(U, B, A, ds)
+        ox, oy = orc.generate_gaussian(N, z)
+        ox, oy = orc.correlate_noise(ox, N), orc.correlate_noise(oy, N)
+        Ao, Bo = orc.prepare_A_and_B_local(ox, oy, Uo, N, 1, 1)
+        Uo = orc.update_uf(Uo, Bo, Ao, ds, N)
+    got = U.download()
+    assert frob_max(got, Uo) <= 1e-12
+    assert frob_max(Uo, U0) > 1e-3
+    c.close()
+
+
+def test_position_evolution_limits(jw):
+    c = jw.Context(512)
+    F = c.field()
+    with pytest.raises(jw.JwError) as e:
+        c.prepare_A_and_B_local(F, c.field(), F, F, F, 3, 1)
+    assert e.value.code in (-1, -4)
+    c.close()
+    c = jw.Context(16)
+    F, G = c.field(), c.field()
+    with pytest.raises(jw.JwError) as e:
+        c.prepare_A_and_B_local(F, G, c.field(), c.field(), c.field(), 2, 1)
+    assert e.value.code == -4
+    c.close()
 
 
 def test_measure_and_dipole(jw, orc):

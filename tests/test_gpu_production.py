@@ -156,8 +156,8 @@ def test_uneven_ensemble_500_steps(jw):
     for t, U1 in fin64.items():
         N, U0 = starts[t]
         c = ctxs[N]
-        c.set_fused([c.FUSED_ROWMAJOR, None, c.FUSED_INTERLEAVED, None, c.FUSED_BLOCK4, None, None, None,
-                     c.FUSED_BLOCK8][lay64[gi[t]]])
+        c.set_fused({0: c.FUSED_ROWMAJOR, 2: c.FUSED_INTERLEAVED, 4: c.FUSED_BLOCK4, 8: c.FUSED_BLOCK8,
+                     16: c.FUSED_ROWMAJOR_NARROW}[lay64[gi[t]]])
         F = c.field(9, U0)
         c.philox(4, t, 0)
         c.evolve(F, jw.FIXED_COUPLING, jw.SIN_KERNEL, 1e-4, nsteps=steps)

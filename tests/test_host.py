@@ -81,6 +81,9 @@ int main(int argc, char** argv) {
     m.set();
     for (int i = 0; i < c.Nx * c.Ny; i += 37) printf("%.17g %.17g %.17g ", m.phat2(i), m.pbarX(i), m.pbarY(i));
     printf("\n");
+    printf("%d %d %d %d %d", (int)c.float_parse, c.mv_zero_mode, c.checkpoint_every, (int)c.resume, c.stop_after);
+    for (auto& g : c.batch) printf(" %d:%d", g.first, g.second);
+    printf("\n");
     return 0;
 }
 ''')
@@ -89,9 +92,31 @@ int main(int argc, char** argv) {
     return exe
 
 
-def run_probe(exe, cfg):
+def run_probe(exe, cfg, extra=False):
     out = subprocess.run([str(exe), str(cfg)], check=True, capture_output=True, text=True).stdout.split("\n")
-    return out[0].split(), [float(v) for v in out[1].split()], [float(v) for v in out[2].split()]
+    res = out[0].split(), [float(v) for v in out[1].split()], [float(v) for v in out[2].split()]
+    return res + (out[3].split(),) if extra else res
+
+
+def test_config_parser_run_keys(host_probe, tmp_path):
+    """float_parse, mv_zero_mode, batch lists, checkpoint_every / resume / stop_after."""
+    cfg = tmp_path / "cfg"
+    base = open(os.path.join(REPO, "tests", "golden", "config_reference")).read()
+    cfg.write_text(base)
+    f, _, _, x = run_probe(host_probe, cfg, extra=True)
+    assert x == ["1", "0", "0", "0", "0"]                   # defaults: the reference's float parse
+    cfg.write_text(base + "float_parse = 0\nmv_zero_mode = 1\ncheckpoint_every = 25\nresume = 1\nstop_after = 60\n")
+    f, _, _, x = run_probe(host_probe, cfg, extra=True)
+    # mu, mass and step read as doubles: the file's decimal values to the last bit
+    assert float(f[1]) == 30.72 and float(f[2]) == 0.0001 and float(f[7]) == 0.0004
+    assert x == ["0", "1", "25", "1", "60"]
+    cfg.write_text(base + "batch = 256:48,512:32,1024:16\n")
+    f, _, _, x = run_probe(host_probe, cfg, extra=True)
+    assert x[5:] == ["256:48", "512:32", "1024:16"] and int(f[13]) == 2     # the batched engine
+    for bad in ("batch = 256\n", "batch = 256:0\n", "batch = 64:2;128:1\n", "mv_zero_mode = 2\n",
+                "checkpoint_every = -1\n"):
+        cfg.write_text(base + bad)
+        assert subprocess.run([str(host_probe), str(cfg)], capture_output=True).returncode == 1, bad
 
 
 def test_config_parser_reference_file(host_probe):

@@ -341,6 +341,29 @@ def test_mv_init_matches_reference_composition(orc, ref):
     assert np.abs(Uo - U).max() < 1e-12
 
 
+@pytest.mark.parametrize("mass", [0.5, 4e-4])
+def test_mv_init_zero_mode_projection(orc, ref, mass):
+    """The projection option sets each sheet's k = 0 mode to zero after solvePoisson:
+    the reference's composition with the k = 0 charge zeroed before its solvePoisson
+    (field.h:867-877, whose factor then multiplies zero).  At m a = 1e-4 (mass 4e-4 in
+    units of g^2 mu = 0.25) the projected start is well conditioned: the oracle's
+    radix-2 FFT and numpy's agree to 1e-12 per site."""
+    N, Ny, g, mu = N64, 3, 1.0, 0.25
+    z = rng.normal(size=Ny * N * N * 8)
+    Uo = orc.mv_init(N, z, Ny, g, mu, mass, zero_mode=1)
+    U = ref.set_unit()
+    for s in range(Ny):
+        f = npfft(orc.set_mv_model(N, z[s * N * N * 8:(s + 1) * N * N * 8], g * mu / np.sqrt(Ny)), N, 1)
+        f[0, :] = 0.0
+        f = npfft(ref.solve_poisson(f, mass * g ** 2 * mu, g), N, 0)
+        U = ref.mul_exp(U, f)
+    assert np.abs(Uo - U).max() < 1e-12
+    m = Uo.reshape(-1, 3, 3)
+    assert np.abs(np.einsum("sij,skj->sik", m, m.conj()) - np.eye(3)).max() < 1e-13
+    # the option changes the start (a plausible no-op implementation fails here)
+    assert np.abs(orc.mv_init(N, z, Ny, g, mu, mass) - Uo).max() > 1e-3
+
+
 def test_measurement_matches_reference_composition(orc, ref):
     U = random_su3_field(orc, N64)
     c = orc.measure_corr(U, N64)
@@ -406,3 +429,80 @@ def test_philox_normals_statistics(orc):
     assert abs(z.mean()) < 0.02 and abs(z.std() - 1) < 0.02
     z2 = orc.philox_normals(N, 7, 0, 1, 0x4C414E47, 16)
     assert abs(np.corrcoef(z, z2)[0, 1]) < 0.02
+
+
+# ---------------------------------------------------------------- POSITION_EVOLUTION
+@pytest.mark.parametrize("coupling", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_prepare_A_and_B_local_matches_reference(orc, ref, coupling, kernel):
+    """The brute-force position-space sums (field.h:2368-2519 at every site) against the
+    reference's own function; its OpenMP reduction fixes no summation order, hence the
+    relative tolerance."""
+    N = N64
+    U = random_su3_field(orc, N)
+    xx, xy = orc.generate_gaussian(N, rng.normal(size=N * N * 16))
+    Ar, Br = ref.prepare_A_and_B_local(xx, xy, U, coupling, kernel)
+    Ao, Bo = orc.prepare_A_and_B_local(xx, xy, U, N, coupling, kernel)
+    assert np.abs(Ao - Ar).max() <= 5e-14 * np.abs(Ar).max()
+    assert np.abs(Bo - Br).max() <= 5e-14 * np.abs(Br).max()
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_prepare_A_and_B_local_is_a_convolution(orc, kernel):
+    """A = w * xi and B = w * (U xi U^dagger) as cyclic convolutions over the
+    displacement x - y (convolution theorem with numpy FFTs), and the SIN weights are
+    the positions.h:131-141 tables: a sign or index slip (y - x for x - y) fails."""
+    N = 16
+    U = random_su3_field(orc, N)
+    xx, xy = orc.generate_gaussian(N, rng.normal(size=N * N * 16))
+    A, B = orc.prepare_A_and_B_local(xx, xy, U, N, 3, kernel)
+    wa, wb = orc.position_weights(N, 3, kernel)
+    vx, vy = orc.uxiulocal(U, xx, xy, N)
+
+    def conv(w, f):
+        W = np.fft.fft2(w.reshape(N, N))
+        F = np.fft.fft2(f.reshape(N, N, 9), axes=(0, 1))
+        return np.fft.ifft2(W[:, :, None] * F, axes=(0, 1)).reshape(N * N, 9)
+
+    assert np.abs(A - (conv(wa, xx) + conv(wb, xy))).max() < 1e-13 * np.abs(A).max()
+    assert np.abs(B - (conv(wa, vx) + conv(wb, vy))).max() < 1e-13 * np.abs(B).max()
+    # the reversed displacement gives a different (sign-flipped, odd kernel) A
+    rev = lambda w: w.reshape(N, N)[(-np.arange(N)) % N][:, (-np.arange(N)) % N].reshape(-1)
+    assert np.abs(A - (conv(rev(wa), xx) + conv(rev(wb), xy))).max() > 1e-2 * np.abs(A).max()
+    if kernel == 1:
+        i = np.arange(N)
+        xh = 0.5 * N * np.sin(2 * np.pi * i / N) / np.pi
+        xb = N * np.sin(np.pi * i / N) / np.pi
+        r2 = xb[:, None] ** 2 + xb[None, :] ** 2
+        with np.errstate(divide="ignore", invalid="ignore"):
+            expect = np.where(r2 > 1e-8, xh[:, None] / r2, 0.0).reshape(-1)
+        assert np.allclose(wa, expect, rtol=1e-14, atol=0)
+
+
+def test_position_kernel_against_momentum_kernel():
+    """Cross-check of the two discretisations of the same kernel (reference facts, no
+    oracle call): the momentum path's kernel -2 pi i pbar/phat2 (field.h:2052-2053)
+    acting through the unitary transforms equals, in position space, the convolution
+    with (1/V) sum_k K(k) e^{ik.r}; the position path uses xhat(r)/xbar2(r)
+    (positions.h:131-141).  Their spectra agree within 16% at every nonzero mode of
+    wavelength 8 or more lattice spacings (|k| <= N/8 per axis), exactly at none: the
+    momentum and position evolutions are different lattice discretisations (they are
+    not expected to match step for step)."""
+    for N in (16, 64):
+        k = np.arange(N)
+        q4 = 4 * np.sin(np.pi * k / N) ** 2
+        sb = np.sin(2 * np.pi * k / N)
+        ph = q4[:, None] + q4[None, :]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            Kx = np.where(ph > 1e-8, -2 * np.pi * 1j * sb[:, None] / ph, 0)
+            xh = 0.5 * N * np.sin(2 * np.pi * k / N) / np.pi
+            xb = N * np.sin(np.pi * k / N) / np.pi
+            r2 = xb[:, None] ** 2 + xb[None, :] ** 2
+            w = np.where(r2 > 1e-8, xh[:, None] / r2, 0.0)
+        Kpos = np.fft.fft2(w)
+        kw = np.abs(np.where(k < N // 2, k, k - N))
+        m = (np.abs(Kx) > 1e-12) & (kw[:, None] <= N // 8) & (kw[None, :] <= N // 8)
+        ratio = Kpos[m] / Kx[m]
+        assert np.abs(ratio.imag).max() < 1e-12
+        assert 0.84 < ratio.real.min() and ratio.real.max() < 1.15
+        assert np.abs(ratio.real - 1).max() > 0.05
