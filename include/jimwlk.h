@@ -197,6 +197,26 @@ int jw_evolve(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, c
 int jw_evolve_mixed(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
                     int nsteps, int reunit_every);
 
+/* ---- allocation-free evolution -------------------------------------------
+ * No counterpart in the reference.  It is for callers that cannot allocate
+ * device memory while they run: code under HIP stream or graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  jw_evolve_noalloc computes
+ * what jw_evolve computes (same kernels, same results) and calls no hipMalloc,
+ * hipFree or allocating library routine: its workspace (the 24 intermediate
+ * planes and the coupling tables, 400 V bytes) is one library-owned device
+ * array per GPU of JW_NOALLOC_SCRATCH_BYTES (100 MiB), held until the process
+ * ends.  The context, the field and any noise injection are made beforehand
+ * (those calls allocate).  Limits: N <= JW_NOALLOC_MAX_N (JW_ERR_ARG above);
+ * the LINEAR kernel is JW_ERR_UNSUPPORTED (its 9-component path needs 54
+ * planes).  Every call on one GPU shares that GPU's array and leaves its
+ * contents behind: calls that overlap on one GPU (two contexts' streams, or
+ * two threads) race, and the caller must serialise them.  Per-kernel timing
+ * (jw_timing) is not recorded in this call. */
+#define JW_NOALLOC_MAX_N 512
+#define JW_NOALLOC_SCRATCH_BYTES (400ull * JW_NOALLOC_MAX_N * JW_NOALLOC_MAX_N)
+int jw_evolve_noalloc(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
+                      int nsteps);
+
 /* ---- batched form ------------------------------------------------------
  * No counterpart in the reference (one lattice per run); it advances many
  * independent trajectories -- the reference's `stat` loop of main_optimized.cpp

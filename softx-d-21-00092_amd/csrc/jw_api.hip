@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 using jw::cplx;
@@ -189,6 +190,43 @@ int evolve_sin(jw_ctx* c, jw_field* U, int coupling, double step, const jw_alpha
     const int rc = evolve_generator_basis(c, U, nsteps, fused, Xi, Vv, ZA, ZB, sa_pos, sa_k, kc, corr, s, ss);
     c->layout = saved_layout;
     return rc;
+}
+
+// ---------------------------------------------------------------- allocation-free evolution
+// Workspace of jw_evolve_noalloc: the 24 intermediate planes (384 V bytes) and
+// two coupling tables of V doubles (16 V bytes), for N <= JW_NOALLOC_MAX_N.
+// One copy per GPU, held until the process ends.  Host code never uses the
+// array's name as an address: each device's copy is reached through the
+// address hipGetSymbolAddress gives for it (scratch_setup, at context
+// creation, with that device current).
+constexpr size_t kScratchBytes = (size_t)400 * JW_NOALLOC_MAX_N * JW_NOALLOC_MAX_N;
+static_assert(kScratchBytes == JW_NOALLOC_SCRATCH_BYTES, "scratch size");
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// once per device, with `device` current: the device address of its scratch copy
+int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(JW_ERR_UNSUPPORTED, "device %d: at most %d devices", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return JW_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(JW_ERR_HIP, "device %d: address of the allocation-free workspace: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+    return JW_OK;
+}
+
+void* scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
 }
 
 // ---------------------------------------------------------------- batches
@@ -387,6 +425,7 @@ int jw_create(jw_ctx** out, int N, int device) {
     HIP_CHECK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(JW_ERR_ARG, "device %d of %d", device, ndev);
     HIP_CHECK(hipSetDevice(device));
+    if (int e = scratch_setup(device)) return e;
     jw_ctx* c = new jw_ctx();
     memset(c, 0, sizeof *c);
     c->N = N;
@@ -827,6 +866,31 @@ int jw_evolve(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, con
     }
     if (int e = ensure_ws(c, 24)) return e;
     return evolve_sin(c, U, coupling, step, a, nsteps, c->ws, nullptr);
+}
+
+int jw_evolve_noalloc(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, const jw_alpha* a,
+                      int nsteps) {
+    if (int e = check_field(U, c, 9, "U")) return e;
+    if (coupling == JW_HATTA_COUPLING_CONSTANT)
+        return fail(JW_ERR_UNSUPPORTED, "HATTA coupling: the reference leaves the coupling uninitialised here");
+    if (coupling < 0 || coupling > 4 || (kernel != 0 && kernel != 1)) return fail(JW_ERR_ARG, "bad coupling/kernel");
+    if (kernel == JW_LINEAR_KERNEL)
+        return fail(JW_ERR_UNSUPPORTED, "the allocation-free evolution runs the SIN kernel");
+    if (c->N > JW_NOALLOC_MAX_N)
+        return fail(JW_ERR_ARG, "N = %d: the allocation-free evolution holds lattices up to N = %d", c->N,
+                    JW_NOALLOC_MAX_N);
+    if (!(step >= 0)) return fail(JW_ERR_ARG, "step must be >= 0");
+    if (nsteps < 0) return fail(JW_ERR_ARG, "nsteps < 0");
+    if (c->inj_per_site && nsteps > 1) return fail(JW_ERR_STATE, "an injection covers one step; nsteps = %d", nsteps);
+    void* ws = scratch_of(c->device);
+    if (!ws) return fail(JW_ERR_STATE, "device %d has no allocation-free workspace", c->device);
+    cplx* Xi = static_cast<cplx*>(ws);
+    double* tabs = reinterpret_cast<double*>(Xi + 24 * c->V);
+    const int saved_timing = c->timing;      // timing events are not created here
+    c->timing = 0;
+    const int rc = evolve_sin(c, U, coupling, step, a, nsteps, Xi, tabs);
+    c->timing = saved_timing;
+    return rc;
 }
 
 int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, const jw_alpha* a, int nsteps,

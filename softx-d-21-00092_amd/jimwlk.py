@@ -71,6 +71,8 @@ def lib():
         L.jw_checkpoint_read.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p]
         L.jw_evolve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 ctypes.POINTER(Alpha), ctypes.c_int]
+        L.jw_evolve_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.POINTER(Alpha), ctypes.c_int]
         L.jw_evolve_mixed.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                       ctypes.POINTER(Alpha), ctypes.c_int, ctypes.c_int]
         L.jw_mv_init.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -264,6 +266,11 @@ class Context:
     # engine
     def evolve(self, U, coupling=FIXED_COUPLING, kernel=SIN_KERNEL, step=1e-4, al=None, nsteps=1):
         _check(lib().jw_evolve(self.h, U.h, coupling, kernel, step, ctypes.byref(al) if al else None, nsteps))
+
+    def evolve_noalloc(self, U, coupling=FIXED_COUPLING, kernel=SIN_KERNEL, step=1e-4, al=None, nsteps=1):
+        """jw_evolve_noalloc: jw_evolve on the library's per-GPU workspace (N <= 512, no allocation)."""
+        _check(lib().jw_evolve_noalloc(self.h, U.h, coupling, kernel, step, ctypes.byref(al) if al else None,
+                                       nsteps))
 
     def evolve_mixed(self, U, coupling=FIXED_COUPLING, kernel=SIN_KERNEL, step=1e-4, al=None, nsteps=1,
                      reunit_every=10):
