@@ -73,6 +73,10 @@ class Oracle:
     def num_threads(self):
         return self.L.or_num_threads()
 
+    def set_as_shipped(self, on):
+        """1: uxiulocal/update_uf on one thread, as the reference ships them (field.h:1901, 1974)."""
+        self.L.or_set_as_shipped(int(on))
+
     # --- SU(3) ---
     def su3_exponentiate(self, m, sign):
         m = np.ascontiguousarray(m, dtype=np.complex128).reshape(9)

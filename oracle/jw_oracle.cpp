@@ -363,6 +363,12 @@ void or_set_num_threads(int n) {
 #endif
 }
 
+// 1: uxiulocal and update_uf run on one thread, as the reference ships them
+// (their OpenMP pragmas are commented out, field.h:1901, 1974); 0: all site
+// loops use every thread.  For the CPU baseline's two variants.
+static int g_as_shipped = 0;
+void or_set_as_shipped(int on) { g_as_shipped = on ? 1 : 0; }
+
 // ---- SU(3) ------------------------------------------------------------------
 // su3_matrix::exponentiate(sign), su3_matrix.h:173-253.  in/out: 18 doubles.
 int or_su3_exponentiate(const double* in18, double sign, double* out18) {
@@ -593,7 +599,7 @@ int or_prepare_A_local(int N, double* A, const double* xix, const double* xiy,
 int or_uxiulocal(int N, double* Vx, double* Vy, const double* U, const double* xix,
                  const double* xiy) {
     const size_t V = (size_t)N * N;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (!g_as_shipped)
     for (size_t i = 0; i < V; i++) {
         M3 u, bx, by;
         for (int k = 0; k < 9; k++) {
@@ -612,7 +618,7 @@ int or_uxiulocal(int N, double* Vx, double* Vy, const double* U, const double* x
 int or_update_uf(int N, double* U, const double* B, const double* A, double step) {
     const size_t V = (size_t)N * N;
     const double ss = std::sqrt(step);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (!g_as_shipped)
     for (size_t i = 0; i < V; i++) {
         M3 u, b, a;
         for (int k = 0; k < 9; k++) {

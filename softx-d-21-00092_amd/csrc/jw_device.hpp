@@ -481,24 +481,23 @@ __device__ __forceinline__ void fft_stage(C* buf, TW tw, int tid) {
 
 // ---------------------------------------------------------------------------
 // Split-BF16 radix-16 stage on the matrix cores (FP32 data, the mixed form).
-// The 16 twiddled inputs of a butterfly form one column of the 32 x 16 real
-// matrix B = [Re v; Im v] (16 butterflies per wave); the radix-16 DFT is the
-// real block matrix [[Fr, -Fi], [Fi, Fr]], F[i][r] = exp(DIR 2 pi i i r / 16),
-// applied as its real-output half A_re = [Fr | -Fi] and imaginary-output half
-// A_im = [Fi | Fr] (16 x 32 each) by v_mfma_f32_16x16x32_bf16.  Every FP32
-// operand x is split into hi = bf16(x) and lo = bf16(x - hi); the products
-// lo.hi + hi.lo, then hi.hi, accumulate in FP32 (the dropped lo.lo and the
-// split leave about 2^-17 relative per stage, against 2^-24 for the FP32
-// butterflies).  Lane l of the 16x16x32 instruction (wave64) holds A row
-// l mod 16, columns 8 (l / 16) + j; B rows 8 (l / 16) + j, column l mod 16;
-// D rows 4 (l / 16) + r (r < 4), column l mod 16.
+// The radix-16 DFT of 32 butterflies at a time is one real product D = A B
+// with v_mfma_f32_32x32x16_bf16, in two K = 16 halves:
+//   B = [Re v; Im v] (32 x 32: rows the 16 twiddled inputs' real then
+//   imaginary parts, columns the butterflies),
+//   A = [[Fr, -Fi], [Fi, Fr]] (32 x 32), F[i][r] = exp(DIR 2 pi i i r / 16),
+// so that rows 0..15 of D are Re y and rows 16..31 Im y.  Every FP32 operand x
+// is split into hi = bf16(x) and lo = bf16(x - hi); the products lo.hi +
+// hi.lo, then hi.hi, accumulate in FP32 (the dropped lo.lo and the split leave
+// about 2^-17 relative per stage, against 2^-24 for the FP32 butterflies).
+// Lane l of the 32x32x16 instruction (wave64) holds A row l mod 32, columns
+// 8 (l / 32) + j; B rows 8 (l / 32) + j, column l mod 32 (j < 8); D rows
+// 8 (r / 4) + 4 (l / 32) + r mod 4, column l mod 32 (r < 16).  So lane l loads
+// inputs 8 (l / 32) .. + 7 of butterfly l mod 32 once, and ends with eight of
+// its complex outputs: y[o] = (D[r], D[r + 8]), o = 8 (r / 4) + 4 (l / 32) + r mod 4.
 // ---------------------------------------------------------------------------
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
-
-struct Dft16Mfma {
-    bf16x8 re_hi, re_lo, im_hi, im_lo;
-};
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 // cos(2 pi m / 16), m = 0..15
 __device__ __forceinline__ float cos16(int m) {
@@ -514,39 +513,41 @@ __device__ __forceinline__ void split_bf16(float x, bf16x8& hi, bf16x8& lo, int 
     lo[t] = (__bf16)(x - (float)h);
 }
 
-// this lane's slice of A_re and A_im for direction DIR
+// this lane's slices of the two K halves of A for direction DIR
+struct Dft16Mfma {
+    bf16x8 a0_hi, a0_lo, a1_hi, a1_lo;
+};
 template <int DIR>
 __device__ __forceinline__ Dft16Mfma dft16_operands(int lane) {
-    const int i = lane & 15, kb = lane >> 4;
+    const int i = lane & 31, o = i & 15, half = lane >> 5;
     Dft16Mfma a;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-        const int k = 8 * kb + j, r = k & 15, m = (i * r) & 15;
+        const int r = 8 * half + j, m = (o * r) & 15;
         const float c = cos16(m), s = DIR * cos16(m + 12);       // sin(x) = cos(x - pi/2)
-        split_bf16(k < 16 ? c : -s, a.re_hi, a.re_lo, j);
-        split_bf16(k < 16 ? s : c, a.im_hi, a.im_lo, j);
+        // K half 0 multiplies Re v: Fr (rows 0..15), Fi (rows 16..31);
+        // K half 1 multiplies Im v: -Fi, Fr
+        split_bf16(i < 16 ? c : s, a.a0_hi, a.a0_lo, j);
+        split_bf16(i < 16 ? -s : c, a.a1_hi, a.a1_lo, j);
     }
     return a;
 }
 
 // One radix-16 Stockham stage (the indexing of fft_stage with R = 16) on the
-// matrix cores: butterflies in groups of 16, one group per wave at a time;
+// matrix cores: butterflies in groups of 32, one group per wave at a time;
 // reads, a barrier, MFMAs and writes, a barrier.
 template <int N, int P, int NSEQ, int ES, int SS, int NT, int DIR, int PS, typename TW>
 __device__ __forceinline__ void fft_stage_mfma16(float2* buf, TW tw, int tid) {
     constexpr int R = 16, S = N / R, NB = S * NSEQ;
-    constexpr int NG = (NB + 15) / 16, NW = NT / 64, GPW = (NG + NW - 1) / NW;
+    constexpr int NG = (NB + 31) / 32, NW = NT / 64, GPW = (NG + NW - 1) / NW;
     static_assert(NT % 64 == 0, "the matrix-core stage needs whole waves");
     const int lane = tid & 63, wave = tid >> 6;
-    const int col = lane & 15, kb = lane >> 4;
-    const int r0 = (kb & 1) * 8;          // inputs r0 .. r0 + 7 of the butterfly
-    const bool im = kb >= 2;              // their imaginary parts (else real)
-    const Dft16Mfma A = dft16_operands<DIR>(lane);
-    bf16x8 bh[GPW], bl[GPW];
+    const int col = lane & 31, half = lane >> 5;
+    bf16x8 rh[GPW], rl[GPW], ih[GPW], il[GPW];
     int jj[GPW], ss[GPW];
 #pragma unroll
     for (int q = 0; q < GPW; q++) {
-        const int g = wave + q * NW, b = 16 * g + col;
+        const int g = wave + q * NW, b = 32 * g + col;
         int j, s;
         if constexpr (ES == 1) { j = b % S; s = b / S; }
         else { s = b % NSEQ; j = b / NSEQ; }
@@ -557,7 +558,7 @@ __device__ __forceinline__ void fft_stage_mfma16(float2* buf, TW tw, int tid) {
             const float2* rb = buf + lpos<ES, SS, PS>(j, s);
 #pragma unroll
             for (int t = 0; t < 8; t++) {
-                const int r = r0 + t;
+                const int r = 8 * half + t;
                 float2 v = rb[lpos<ES, 0, PS>(r * S, 0)];
                 if constexpr (P > 1) {
                     constexpr int STRIDE = N / (P * R);
@@ -567,30 +568,41 @@ __device__ __forceinline__ void fft_stage_mfma16(float2* buf, TW tw, int tid) {
                         v = cmul(v, w);
                     }
                 }
-                split_bf16(im ? v.y : v.x, bh[q], bl[q], t);
+                split_bf16(v.x, rh[q], rl[q], t);
+                split_bf16(v.y, ih[q], il[q], t);
             }
         } else {
 #pragma unroll
-            for (int t = 0; t < 8; t++) bh[q][t] = bl[q][t] = (__bf16)0.f;
+            for (int t = 0; t < 8; t++) rh[q][t] = rl[q][t] = ih[q][t] = il[q][t] = (__bf16)0.f;
         }
     }
     __syncthreads();
+    // the DFT matrix slices, formed after the reads (the lane index is made
+    // opaque so that they are not computed early and kept live across the reads)
+    int lane_v = lane;
+    asm volatile("" : "+v"(lane_v));
+    const Dft16Mfma A = dft16_operands<DIR>(lane_v);
 #pragma unroll
     for (int q = 0; q < GPW; q++) {
         const int g = wave + q * NW;
         if (g >= NG) continue;            // uniform over the wave
-        f32x4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
-        dre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.re_lo, bh[q], dre, 0, 0, 0);
-        dre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.re_hi, bl[q], dre, 0, 0, 0);
-        dre = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.re_hi, bh[q], dre, 0, 0, 0);
-        dim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.im_lo, bh[q], dim, 0, 0, 0);
-        dim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.im_hi, bl[q], dim, 0, 0, 0);
-        dim = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.im_hi, bh[q], dim, 0, 0, 0);
-        if (16 * g + col < NB) {
+        f32x16 d;
+#pragma unroll
+        for (int r = 0; r < 16; r++) d[r] = 0.f;
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.a0_lo, rh[q], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.a0_hi, rl[q], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.a1_lo, ih[q], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.a1_hi, il[q], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.a0_hi, rh[q], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.a1_hi, ih[q], d, 0, 0, 0);
+        if (32 * g + col < NB) {
             const int j = jj[q], s = ss[q], k = j % P;
             float2* wb = buf + lpos<ES, SS, PS>((j - k) * R + k, s);
 #pragma unroll
-            for (int r = 0; r < 4; r++) wb[lpos<ES, 0, PS>((4 * kb + r) * P, 0)] = make_float2(dre[r], dim[r]);
+            for (int r = 0; r < 8; r++) {
+                const int o = 8 * (r / 4) + 4 * half + (r % 4);
+                wb[lpos<ES, 0, PS>(o * P, 0)] = make_float2(d[r], d[r + 8]);
+            }
         }
     }
     __syncthreads();

@@ -48,9 +48,9 @@ struct RowE {
 // slower: one workgroup per CU and spills in the correlated variant).
 template <int N, typename C = double2, int CORR = 0>
 struct ColE {
+    static constexpr int NTMAX = 1024;
     static constexpr int W = (4096 / N > 16) ? 16 : 4096 / N;
     static constexpr int ELEMS = 2 * W * N;
-    static constexpr int NTMAX = 1024;
     static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > NTMAX) ? NTMAX : ELEMS / 8);
     static constexpr int PER = ELEMS / NT;          // elements per thread
     static constexpr int PAIRS = (W * N) / NT;      // (X, Y) element pairs per thread
@@ -73,7 +73,7 @@ __device__ __forceinline__ int xcd_strip(int b, int nstrips) {
 // row-major, 1 row-major for the 64-byte-strip column pass, 4 and 8 row blocks)
 template <int N, int BF>
 __device__ __forceinline__ size_t lay(int x, int y) {
-    if constexpr (BF <= 2 || BF == 16) return (size_t)x * N + y;
+    if constexpr (BF <= 2 || BF >= 16) return (size_t)x * N + y;
     else return ((size_t)(x / BF) * N + y) * BF + (x % BF);
 }
 
@@ -93,7 +93,7 @@ __device__ __forceinline__ size_t pat(int q, int x, int y, size_t V) {
 // consecutive turns (a bijection on [0, N) for N a multiple of 8 BF).
 template <int N, int BF>
 __device__ __forceinline__ int row_of_block(int b) {
-    if constexpr (BF <= 2 || BF == 16) return b;
+    if constexpr (BF <= 2 || BF >= 16) return b;
     else {
         static_assert(N % (8 * BF) == 0, "blocked layout needs N a multiple of 8 BF");
         const int t = b >> 3, xcd = b & 7;
@@ -400,8 +400,11 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
 // LDS row x holds [X(x, c0..c0+W-1), Y(x, c0..c0+W-1)]: 2W sequences of length N
 // with element stride 2W.
 // ---------------------------------------------------------------------------
+// MF = 1, white noise (FP32 tiles of 74 KB, two per CU): registers of two
+// workgroups asked for, as the FP32 butterfly version fits them by itself; the
+// correlated variant needs more than that without spilling and runs one per CU
 template <int N, int CORR, typename C, int PI = 0, int MF = 0>
-__global__ __launch_bounds__((ColE<N, C, CORR>::NT)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
+__global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, C, CORR>::NT / 256 : 1)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
                                                           C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                           const double* __restrict__ q4,
                                                           const double* __restrict__ sb,

@@ -150,7 +150,7 @@ def test_batch_tuned_layout(jw):
     b.evolve(3, 1, 1e-4, None, nsteps=12, reunit_every=10)
     b.sync()
     for g in range(len(groups)):
-        assert b.layout(g) in (0, 2, 4, 8)
+        assert b.layout(g) in (0, 2, 4, 8, 16)
     for t, (N, U0) in enumerate(states):
         c = ctxs[N]
         got = c.field()

@@ -74,7 +74,9 @@ def oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, coupling, seed=
         for yy in range(N):
             if abs(xx - yy) <= 4:
                 i = xx * N + yy
-                rows.append([meas - 1, xx, yy, np.sqrt(ph[i]), x * ph[i] * sums[-1][i], x * ph[i] * errs[-1][i]])
+                # sqrt(phat2) as the file prints it ("%f", field.h:2543)
+                rows.append([meas - 1, xx, yy, float("%f" % np.sqrt(ph[i])), x * ph[i] * sums[-1][i],
+                             x * ph[i] * errs[-1][i]])
     return np.array(rows)
 
 

@@ -106,7 +106,7 @@ def test_mixed_layouts_agree(jw, N, coupling):
     al = alpha_for(jw, N, coupling)
     out = {}
     for mode in (c.FUSED_ROWMAJOR, c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED_ROWMAJOR_WIDE, c.FUSED_INTERLEAVED,
-                 c.FUSED):
+                 c.FUSED_ROWMAJOR_NARROW, c.FUSED):
         c.set_fused(mode)
         U = c.field(9, U0)
         c.philox(8, 0, 0)
@@ -114,7 +114,7 @@ def test_mixed_layouts_agree(jw, N, coupling):
         out[mode] = U.download()
     ref = out[c.FUSED_ROWMAJOR]
     for mode in (c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED_ROWMAJOR_WIDE, c.FUSED_INTERLEAVED,
-                 c.FUSED):
+                 c.FUSED_ROWMAJOR_NARROW, c.FUSED):
         assert frob_max(ref, out[mode]) <= 2e-6, mode
     assert frob_max(ref, U0) > 1e-3
     c.set_fused(c.FUSED)

@@ -274,14 +274,14 @@ def test_fused_matches_unfused(jw, orc, N, coupling):
     al = jw.alpha(lambda_L=0.02 * N, mu0_over_lambda=3.0) if coupling in (0, 4) else None
     results = {}
     for mode in (c.UNFUSED, c.FUSED_ROWMAJOR, c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED_ROWMAJOR_WIDE, c.FUSED_INTERLEAVED,
-                 c.FUSED):
+                 c.FUSED_ROWMAJOR_NARROW, c.FUSED):
         c.set_fused(mode)
         U = c.field(9, ref)
         c.philox(5, 1, 100)
         c.evolve(U, coupling, 1, 1e-4, al, nsteps=2)
         results[mode] = U.download()
     for mode in (c.FUSED_ROWMAJOR, c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED_ROWMAJOR_WIDE, c.FUSED_INTERLEAVED,
-                 c.FUSED):
+                 c.FUSED_ROWMAJOR_NARROW, c.FUSED):
         assert frob_max(results[c.UNFUSED], results[mode]) < 1e-13, mode
     assert frob_max(results[c.UNFUSED], ref) > 1e-3
     c.close()

@@ -45,7 +45,34 @@ def mv_start(c, Ny, g2mu, seed=1, traj=0):
     return U
 
 
+def npfft(f, N, d):
+    """The reference's unitary transform (mpi_fftw_class.h:191-315) through numpy."""
+    a = f.reshape(N, N, -1)
+    out = np.fft.fft2(a, axes=(0, 1)) / N if d == 1 else np.fft.ifft2(a, axes=(0, 1)) * N
+    return out.reshape(N * N, -1)
+
+
+def numpy_step(orc, U, z, N, coupling, ds, al_o):
+    """A second FP64 implementation of the step: the oracle's per-site functions
+    with numpy's FFT instead of the oracle's radix-2 one."""
+    xx, xy = orc.generate_gaussian(N, z)
+    fx, fy = npfft(xx, N, 1), npfft(xy, N, 1)
+    if coupling == 4:
+        fx, fy = orc.scale_by_sqrt_alpha(fx, N, al_o), orc.scale_by_sqrt_alpha(fy, N, al_o)
+        xx, xy = npfft(fx, N, 0), npfft(fy, N, 0)
+    A = npfft(orc.prepare_A_local(fx, fy, N, 3 if coupling == 4 else coupling, 1), N, 0)
+    vx, vy = orc.uxiulocal(U, xx, xy, N)
+    B = npfft(orc.prepare_A_local(npfft(vx, N, 1), npfft(vy, N, 1), N, 3 if coupling == 4 else coupling, 1), N, 0)
+    return orc.update_uf(U, B, A, ds, N)
+
+
 def test_correlated_step_N2048_against_oracle(jw, orc):
+    """One injected FP64 step at N = 2048 against the oracle, per-site Frobenius.
+    The kernel's 1/|k| infrared weight (up to N / 2 pi) amplifies the rounding of
+    the transforms, so two correct FP64 implementations of the step -- the
+    oracle's radix-2 FFT and numpy's -- differ by about 4e-12 here (1e-13 at
+    N = 256).  The HIP step must lie within 1e-12 of the oracle or, where the
+    two CPU implementations are further apart than that, within their spread."""
     N = 2048
     print(f"production: N={N} correlated FP64 injected step", flush=True)
     al_g, al_o = alphas(jw, N)
@@ -58,8 +85,10 @@ def test_correlated_step_N2048_against_oracle(jw, orc):
     got = U.download()
     c.close()
     ref = orc.langevin_step(U0, z, N, 4, 1, 5e-5, al_o)
+    spread = frob_max(ref, numpy_step(orc, U0, z, N, 4, 5e-5, al_o))
     err = frob_max(got, ref)
-    assert err <= 1e-12, err
+    print(f"production: |hip - oracle| = {err:.2e}, |numpy - oracle| = {spread:.2e}", flush=True)
+    assert err <= max(1e-12, spread), (err, spread)
     assert frob_max(ref, U0) > 1e-3
 
 
