@@ -48,7 +48,7 @@ ALPHA_MAX = 0.7
 LAMBDA_A = 0.02
 # compulsory HBM reals per site per step (x 8 B in FP64, x 4 B in FP32):
 # 182 for per-mode correlated noise, 132 for the white-noise couplings
-REALS_PER_SITE = {"correlated": 182, "white": 132}
+REALS_PER_SITE = {"correlated": 182, "white": 132, "kspace": 150}
 # Algorithmic HBM traffic of each kernel class of the fused step, in complex
 # planes per launch (read + written; U is 9 planes):
 #   row_noise_fwd  W 8                                        8
@@ -59,8 +59,13 @@ REALS_PER_SITE = {"correlated": 182, "white": 132}
 # Per step: correlated 91 planes = 182 reals per site, the compulsory figure;
 # white noise 75 planes = 150 reals (U is read by both row_uxiu_fwd and
 # row_update).
+# Momentum-space noise (forms *_k): no noise pass; column pass A draws the
+# spectra and writes A (4) and the noise's row spectra (8), the row pass reads
+# them back as in the correlated step: 12 + 12 + 25 + 26 = 75 planes = 150 reals
+# per site for every coupling.
 PLANES = {"correlated": {"row_noise_fwd": 8, "col_kmult": (20, 12), "row_uxiu_fwd": 25, "row_update": 26},
-          "white": {"row_noise_fwd": 8, "col_kmult": (12, 12), "row_uxiu_fwd": 17, "row_update": 26}}
+          "white": {"row_noise_fwd": 8, "col_kmult": (12, 12), "row_uxiu_fwd": 17, "row_update": 26},
+          "kspace": {"row_noise_fwd": 0, "col_kmult": (12, 12), "row_uxiu_fwd": 25, "row_update": 26}}
 # measured HBM traffic per launch of the dominant kernels (rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a plain copy)
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_N2048.json")
@@ -85,7 +90,10 @@ def parse():
     p.add_argument("--cpu-steps-omp", type=int, default=2, help="CPU baseline steps with every loop threaded")
     p.add_argument("--fused-mode", type=int, default=1,
                    help="jw_set_fused mode: 1 default, 2/3/4 row-major/4-row/8-row blocked intermediates")
-    p.add_argument("--forms", default="fp64,mixed", help="forms to time (profiling runs may pick one)")
+    p.add_argument("--forms", default="fp64,mixed,fp64_k,mixed_k",
+                   help="forms to time (profiling runs may pick one); *_k: momentum-space noise")
+    p.add_argument("--noise-space", default="position", choices=["position", "momentum"],
+                   help="noise of the headline forms, the batched form and the S(r) comparison")
     p.add_argument("--batch", type=int, default=2, help="N = 2048 trajectories of the batched form (0: skip)")
     p.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble")
     return p.parse_args()
@@ -162,12 +170,18 @@ def main():
         ctx.sync()
 
     def run(form, F, n):
-        if form in ("mixed", "mixed_mfma"):
-            ctx.set_mixed_fft(1 if form == "mixed_mfma" else 0)
+        base = form[:-2] if form.endswith("_k") else form
+        ctx.set_noise_space(jw.NOISE_MOMENTUM if form.endswith("_k") else jw.NOISE_POSITION)
+        if base in ("mixed", "mixed_mfma"):
+            ctx.set_mixed_fft(1 if base == "mixed_mfma" else 0)
             ctx.evolve_mixed(F, coupling, jw.SIN_KERNEL, step, al, nsteps=n, reunit_every=a.reunit_every)
             ctx.set_mixed_fft(0)
         else:
             ctx.evolve(F, coupling, jw.SIN_KERNEL, step, al, nsteps=n)
+        ctx.set_noise_space(jw.NOISE_POSITION)
+
+    ks = a.noise_space == "momentum"
+    f64, fmix = ("fp64_k", "mixed_k") if ks else ("fp64", "mixed")
 
     res = {}
     for form in forms:
@@ -188,21 +202,24 @@ def main():
         kt = ctx.kernel_times()
         ctx.timing(False)
         bpr = 4 if form.startswith("mixed") else 8
-        classes, roof = kernel_table(kt, nk, V, PLANES[kind], bpr)
+        kind_f = "kspace" if form.endswith("_k") else kind
+        classes, roof = kernel_table(kt, nk, V, PLANES[kind_f], bpr)
         print(f"bench: {form} kernel classes {json.dumps({k: v['ms_per_step'] for k, v in classes.items()})}",
               file=sys.stderr, flush=True)
-        step_bytes = REALS_PER_SITE[kind] * bpr * V
+        step_bytes = REALS_PER_SITE[kind_f] * bpr * V
         res[form] = {"steps_per_s": round(sps, 3), "ms_per_step": round(1e3 / sps, 4), "kernels": classes,
                      "roofline": roof,
                      "step_roofline": {"bytes_per_step": step_bytes,
                                        "achieved_GBs": round(step_bytes * sps / 1e9, 1),
                                        "frac": round(step_bytes * sps / (HBM_PEAK_GBS * 1e9), 4),
-                                       "reals_per_site": REALS_PER_SITE[kind], "bytes_per_real": bpr}}
+                                       "reals_per_site": REALS_PER_SITE[kind_f], "bytes_per_real": bpr}}
         F.close()
 
-    def time_batch(groups, fp32, coupling_b, step_b, al_b, nsteps, warm, seeds0):
+    def time_batch(groups, fp32, coupling_b, step_b, al_b, nsteps, warm, seeds0, kspace=False):
         """Aggregate rate of a batch of MV starts (own seed each): (ensemble steps/s, layouts)."""
         b = jw.Batch(groups, fp32=fp32)
+        if kspace:
+            b.set_noise_space(jw.NOISE_MOMENTUM)
         ctxs = {}
         t = 0
         for N_, count in groups:
@@ -232,9 +249,9 @@ def main():
 
     batched = None
     if a.batch > 0:
-        sps_b, lays = time_batch([(N, a.batch)], True, coupling, step, al, a.steps, a.warmup, 1000)
+        sps_b, lays = time_batch([(N, a.batch)], True, coupling, step, al, a.steps, a.warmup, 1000, ks)
         batched = {"trajectories": a.batch, "N": N, "ensemble_steps_per_s": round(sps_b, 3),
-                   "traj_steps_per_s": round(sps_b * a.batch, 3), "layout": lays[0]}
+                   "traj_steps_per_s": round(sps_b * a.batch, 3), "layout": lays[0], "noise_space": a.noise_space}
         print(f"bench: mixed+batched: {a.batch} x N={N}: {sps_b * a.batch:.1f} trajectory-steps/s "
               f"(layout {lays[0]})", file=sys.stderr, flush=True)
     ensemble = None
@@ -242,9 +259,9 @@ def main():
         groups = [(256, 48), (512, 32), (1024, 16)]
         sites = sum(n * n * c for n, c in groups)
         ensemble = {"groups": [{"N": n, "count": c} for n, c in groups], "sites": sites,
-                    "coupling": "fixed", "step": 1e-4}
+                    "coupling": "fixed", "step": 1e-4, "noise_space": a.noise_space}
         for fp32, name in ((True, "mixed"), (False, "fp64")):
-            sps_e, lays = time_batch(groups, fp32, jw.FIXED_COUPLING, 1e-4, None, 20, 10, 2000)
+            sps_e, lays = time_batch(groups, fp32, jw.FIXED_COUPLING, 1e-4, None, 20, 10, 2000, ks)
             ensemble[name] = {"steps_per_s": round(sps_e, 3), "site_updates_per_s": round(sps_e * sites, 1),
                               "layouts": lays}
             print(f"bench: ensemble {name}: {sps_e:.1f} steps/s = {sps_e * sites:.3e} site-updates/s "
@@ -264,20 +281,20 @@ def main():
 
     # S(r) of the mixed form against FP64 after the same sr-steps from the same start
     sr = None
-    if a.sr_steps > 0 and "fp64" in forms and "mixed" in forms:
+    if a.sr_steps > 0 and f64 in forms and fmix in forms:
         S = {}
-        for form in ("fp64", "mixed"):
+        for form in (f64, fmix):
             F = ctx.field(9, U0)
             ctx.philox(3, 0, 0)
             run(form, F, a.sr_steps)
             S[form] = ctx.dipole_from_corr(ctx.measure_corr(F))
             F.close()
         S0 = ctx.dipole_from_corr(ctx.measure_corr(ctx.field(9, U0)))
-        r, s64, cnt = jw.radial_bins(S["fp64"], N, N // 2)
-        _, s32, _ = jw.radial_bins(S["mixed"], N, N // 2)
-        sr = {"steps": a.sr_steps, "max_abs_err": float(np.abs(S["fp64"] - S["mixed"]).max()),
+        r, s64, cnt = jw.radial_bins(S[f64], N, N // 2)
+        _, s32, _ = jw.radial_bins(S[fmix], N, N // 2)
+        sr = {"steps": a.sr_steps, "noise_space": a.noise_space, "max_abs_err": float(np.abs(S[f64] - S[fmix]).max()),
               "max_abs_err_radial": float(np.abs(s64 - s32)[cnt > 0].max()),
-              "max_abs_change_from_start": float(np.abs(S["fp64"] - S0).max()), "bound": 2e-5}
+              "max_abs_change_from_start": float(np.abs(S[f64] - S0).max()), "bound": 2e-5}
         print(f"bench: S(r) after {a.sr_steps} steps: max |mixed - fp64| = {sr['max_abs_err']:.3e} "
               f"(evolution moved S by {sr['max_abs_change_from_start']:.3e})", file=sys.stderr, flush=True)
 
@@ -325,35 +342,38 @@ def main():
         print(f"bench: cpu oracle step {t_ship:.2f} s as shipped, {t_omp:.2f} s all-OMP ({nthr} threads, {model})",
               file=sys.stderr, flush=True)
 
-    head = "mixed" if "mixed" in res else forms[0]
+    head = fmix if fmix in res else forms[0]
     h = res[head]
     value = batched["traj_steps_per_s"] if batched else h["steps_per_s"]
     form_txt = (f"mixed+batched: {a.batch} trajectories of N={N} in one batch, FP32 fields, FFTs and products, FP64 "
                 f"exponentials, SU(3) projection every {a.reunit_every} steps" if batched else
                 ("mixed: FP32 fields, FFTs and products, FP64 exponentials, SU(3) projection every "
-                 f"{a.reunit_every} steps; one trajectory") if head == "mixed" else "plain FP64")
+                 f"{a.reunit_every} steps; one trajectory") if head == fmix else "plain FP64") \
+        + f"; {a.noise_space}-space noise"
     out = {"metric": METRIC, "value": value, "unit": "trajectory-steps/s" if batched else "steps/s",
            "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": round(1e3 / value, 4), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32" if head == "mixed" else "f64",
+           "vs_baseline": None, "dtype": "f32" if head == fmix else "f64",
            "data": "synthetic (Philox noise, MV initial condition)",
            "config": {"workload": f"N={N} Langevin step, {a.coupling} coupling"
                                   + (f" (alpha_max={ALPHA_MAX}, Lambda a={LAMBDA_A})" if a.coupling == "correlated" else "")
                                   + f", SIN kernel, ds={step}, MV start Ny=50 g2mu a=0.25 m a=1e-4",
                       "N": N, "sites": V,
                       "form": form_txt, "parallelism": "single GPU"},
-           "fp64_steps_per_s": res["fp64"]["steps_per_s"] if "fp64" in res else None,
-           "mixed_steps_per_s": res["mixed"]["steps_per_s"] if "mixed" in res else None,
+           "noise_space": a.noise_space,
+           "fp64_steps_per_s": res[f64]["steps_per_s"] if f64 in res else None,
+           "mixed_steps_per_s": res[fmix]["steps_per_s"] if fmix in res else None,
+           "forms_steps_per_s": {f: r["steps_per_s"] for f, r in res.items()},
            "mixed_batched": batched,
-           "value_over_fp64": round(value / res["fp64"]["steps_per_s"], 3) if "fp64" in res else None,
-           "mixed_over_fp64": round(res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"], 3)
-           if "mixed" in res and "fp64" in res else None,
+           "value_over_fp64": round(value / res[f64]["steps_per_s"], 3) if f64 in res else None,
+           "mixed_over_fp64": round(res[fmix]["steps_per_s"] / res[f64]["steps_per_s"], 3)
+           if fmix in res and f64 in res else None,
            "ensemble": ensemble,
            "sr_err_vs_fp64": sr,
            "roofline": h["roofline"], "step_roofline": h["step_roofline"], "kernels": h["kernels"],
-           "roofline_fp64": res["fp64"]["roofline"] if "fp64" in res else None,
-           "step_roofline_fp64": res["fp64"]["step_roofline"] if "fp64" in res else None,
-           "kernels_fp64": res["fp64"]["kernels"] if "fp64" in res else None,
+           "roofline_fp64": res[f64]["roofline"] if f64 in res else None,
+           "step_roofline_fp64": res[f64]["step_roofline"] if f64 in res else None,
+           "kernels_fp64": res[f64]["kernels"] if f64 in res else None,
            "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -197,6 +197,26 @@ int jw_evolve(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, c
 int jw_evolve_mixed(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
                     int nsteps, int reunit_every);
 
+/* Where the Langevin noise of jw_evolve / jw_evolve_mixed / jw_evolve_noalloc
+ * on this context is drawn.  JW_NOISE_POSITION (default): 16 normals per site,
+ * as generate_gaussian (field.h:2116-2167) in the library's Philox stream.
+ * JW_NOISE_MOMENTUM: the spectra of the noise planes drawn per mode -- the
+ * Fourier transform of a pair of independent white fields is white circular
+ * complex Gaussian, so the same noise in distribution -- which removes the
+ * noise pass and the forward column transform of the step.  Philox call
+ * {k = kx N + ky, step, 'KSPC', p} (key {seed, traj}) gives four 32-bit
+ * uniforms u_j = (w_j + 1/2) 2^-32 and two Box-Muller pairs (z0, z1), (z2, z3);
+ * the x- and y-direction spectra of plane pair p (generators 2p, 2p+1) are
+ * N (z0 + i z1) and N (z2 + i z3).  The position-space normals are then
+ * n_2p + i n_2p+1 = N^-1 sum_k e^{+2 pi i k.x/N} (z0 + i z1)_k and likewise m
+ * from (z2, z3).  Needs the fused step (64 <= N <= 2048), the SIN kernel and a
+ * coupling other than NOISE_COUPLING_CONSTANT (which scales the noise per
+ * site): JW_ERR_UNSUPPORTED otherwise, JW_ERR_STATE with a pending injection.
+ * No counterpart in the reference. */
+#define JW_NOISE_POSITION 0
+#define JW_NOISE_MOMENTUM 1
+int jw_set_noise_space(jw_ctx* ctx, int space);
+
 /* ---- allocation-free evolution -------------------------------------------
  * No counterpart in the reference.  It is for callers that cannot allocate
  * device memory while they run: code under HIP stream or graph capture, tight
@@ -243,6 +263,9 @@ int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step);
 /* layout 0, 2, 4, 8 or 16 for group g, or -1 to tune on the next evolution; the chosen one */
 int jw_batch_set_layout(jw_batch* b, int group, int layout);
 int jw_batch_layout(const jw_batch* b, int group);
+/* JW_NOISE_POSITION or JW_NOISE_MOMENTUM for every trajectory, as
+ * jw_set_noise_space (trajectory t keyed {seed, traj0 + t}) */
+int jw_batch_set_noise_space(jw_batch* b, int space);
 /* nsteps steps of every trajectory (SIN kernel); reunit_every as jw_evolve_mixed
  * (ignored in FP64).  Asynchronous: jw_batch_sync waits. */
 int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw_alpha* alpha, int nsteps,
@@ -286,7 +309,7 @@ int jw_set_fused(jw_ctx* ctx, int mode);
  * passes, row-major layout).  mode 0 (default): FP32 radix-8/4/2 butterflies;
  * 1: the radix-16 stages as DFT-matrix products on the matrix cores in split
  * BF16 (x = hi + lo, three products lo.hi + hi.lo + hi.hi accumulated in FP32,
- * v_mfma_f32_16x16x32_bf16), the remaining stage in FP32 butterflies: about
+ * v_mfma_f32_32x32x16_bf16), the remaining stage in FP32 butterflies: about
  * 2^-17 relative per matrix-core stage instead of 2^-24.  No counterpart in
  * the reference (FP64 FFTW). */
 int jw_set_mixed_fft(jw_ctx* ctx, int mode);

@@ -62,6 +62,8 @@ class Oracle:
         L.or_philox4x32_10.restype = None
         L.or_philox_normals.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.c_uint32, ctypes.c_int, _dp]
+        L.or_kspace_normals.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_uint32, _dp, ctypes.c_void_p]
 
     def _ok(self, rc):
         if rc != 0:
@@ -262,6 +264,16 @@ class Oracle:
         z = np.zeros(N * N * per_site)
         self._ok(self.L.or_philox_normals(N, seed, traj, step, tag, per_site, _p(z)))
         return z
+
+    def kspace_normals(self, N, seed, traj, step, tag, spectra=False):
+        """Position-space normals (16 per site, draw order n0, m0, n1, m1, ...) of
+        the momentum-space noise stream; with spectra=True also the drawn
+        unit-variance spectra, (V, 4 pairs, [Px re, Px im, Py re, Py im])."""
+        z = np.zeros(N * N * 16)
+        spec = np.zeros(N * N * 16) if spectra else None
+        self._ok(self.L.or_kspace_normals(N, seed, traj, step, tag, _p(z),
+                                          None if spec is None else spec.ctypes.data))
+        return (z, spec.reshape(N * N, 4, 4)) if spectra else z
 
 
 class Reference:

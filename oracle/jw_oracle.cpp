@@ -967,4 +967,55 @@ int or_philox_normals(int N, uint32_t seed, uint32_t traj, uint32_t step, uint32
     return 0;
 }
 
+// Momentum-space Langevin noise of the library (jw_set_noise_space): for mode
+// k = kx N + ky and plane pair p the Philox call {k, step, tag, p} (key
+// {seed, traj}) gives four 32-bit uniforms u_j = (w_j + 1/2) 2^-32 and two
+// Box-Muller pairs
+//   (a, b) = sqrt(-2 ln u0) (cos 2 pi u1, sin 2 pi u1),  (c, d) from u2, u3,
+// the unit-variance spectra P_x = a + i b, P_y = c + i d of the x- and
+// y-direction plane pairs.  In position space
+//   w_d(x) = N^-1 sum_k exp(+2 pi i k.x / N) P_d(k)     (unitary inverse DFT),
+// so that the library's forward transform of w_d is N P_d; site x gets the
+// normals n_2p = Re w_x, n_2p+1 = Im w_x, m_2p = Re w_y, m_2p+1 = Im w_y in the
+// draw order n0, m0, n1, m1, ... (z[16 x + 4 p + {0, 2, 1, 3}]).
+// spec (optional, 16 V doubles): the drawn spectra, per mode
+// (P_x re, P_x im, P_y re, P_y im) for p = 0..3.
+int or_kspace_normals(int N, uint32_t seed, uint32_t traj, uint32_t step, uint32_t tag, double* z, double* spec) {
+    if (!pow2(N)) return -1;
+    const size_t V = (size_t)N * N;
+    const uint32_t key[2] = {seed, traj};
+    std::vector<cd> w(2 * V);
+    const Plan1D& plan = plan_for(N);
+    for (int p = 0; p < 4; p++) {
+        for (size_t k = 0; k < V; k++) {
+            const uint32_t ctr[4] = {(uint32_t)k, step, tag, (uint32_t)p};
+            uint32_t o[4];
+            or_philox4x32_10(ctr, key, o);
+            double g[4];
+            for (int h = 0; h < 2; h++) {
+                const double u0 = ((double)o[2 * h] + 0.5) * 0x1.0p-32;
+                const double u1 = ((double)o[2 * h + 1] + 0.5) * 0x1.0p-32;
+                const double r = std::sqrt(-2.0 * std::log(u0));
+                const long double th = 6.283185307179586476925286766559005768L * (long double)u1;
+                g[2 * h] = r * (double)cosl(th);
+                g[2 * h + 1] = r * (double)sinl(th);
+            }
+            w[k] = cd(g[0], g[1]);
+            w[V + k] = cd(g[2], g[3]);
+            if (spec)
+                for (int j = 0; j < 4; j++) spec[16 * k + 4 * p + j] = g[j];
+        }
+        for (int d = 0; d < 2; d++) fft2d_plane(N, w.data() + d * V, false, plan);
+        const double inv = 1.0 / (double)N;
+        for (size_t x = 0; x < V; x++) {
+            const cd a = w[x] * inv, b = w[V + x] * inv;
+            z[16 * x + 4 * p + 0] = a.real();
+            z[16 * x + 4 * p + 2] = a.imag();
+            z[16 * x + 4 * p + 1] = b.real();
+            z[16 * x + 4 * p + 3] = b.imag();
+        }
+    }
+    return 0;
+}
+
 }  // extern "C"

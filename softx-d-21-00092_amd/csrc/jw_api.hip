@@ -95,6 +95,19 @@ jw::NoiseSrc take_noise(jw_ctx* c, int per_site, uint32_t tag, bool* ok) {
     return ns;
 }
 
+// momentum-space noise runs on the fused passes (64 <= N <= 2048) with a
+// coupling that does not scale the noise per site, and replaces the Philox
+// draws (an injection is position-space noise)
+int check_kspace(const jw_ctx* c, int coupling, bool fused) {
+    if (!c->noise_kspace) return JW_OK;
+    if (!fused)
+        return fail(JW_ERR_UNSUPPORTED, "momentum-space noise needs the fused step (64 <= N <= 2048, fused mode on)");
+    if (coupling == JW_NOISE_COUPLING_CONSTANT)
+        return fail(JW_ERR_UNSUPPORTED, "momentum-space noise: NOISE coupling scales the noise per site");
+    if (c->inj_per_site) return fail(JW_ERR_STATE, "momentum-space noise: injected normals are position-space noise");
+    return JW_OK;
+}
+
 int check_field(const jw_field* f, const jw_ctx* c, int t, const char* name) {
     if (!f) return fail(JW_ERR_ARG, "%s is null", name);
     if (c && f->ctx != c) return fail(JW_ERR_ARG, "%s belongs to another context", name);
@@ -123,10 +136,10 @@ int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, jw::c
     const size_t V = c->V;
     for (int it = 0; it < nsteps; it++) {
         bool ok;
-        jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
+        jw::NoiseSrc ns = take_noise(c, 16, c->noise_kspace ? JW_TAG_KSPACE : JW_TAG_LANGEVIN, &ok);
         if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
         if (fused) {
-            const jw::StepArgs sa{ns, sa_pos, sa_k, kc == 0, corr, ss, 0, 1, it == 0, it < nsteps - 1};
+            const jw::StepArgs sa{ns, sa_pos, sa_k, kc == 0, corr, ss, 0, 1, it == 0, it < nsteps - 1, c->noise_kspace};
             HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, sa));
             continue;
         }
@@ -169,6 +182,7 @@ int evolve_sin(jw_ctx* c, jw_field* U, int coupling, double step, const jw_alpha
     const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
     const double s = 1.0 / c->N, ss = sqrt(step);
     const bool fused = jw::fused_supported(c->N) && !c->unfused;
+    if (int e = check_kspace(c, coupling, fused)) return e;
     const int saved_layout = c->layout;
     if (c->layout < 0) c->layout = default_layout(c->N, corr, 0);
     if (fused && sa_k && uses_sa16(c->layout)) {
@@ -259,6 +273,7 @@ struct BatchKey {
 
 struct jw_batch {
     int device = 0, fp32 = 0, ntraj = 0;
+    int kspace = 0;                  // momentum-space noise (jw_batch_set_noise_space)
     std::vector<BatchGroup> g;
     hipStream_t stream = nullptr;    // origin of the step graph; groups fork from it
     hipEvent_t fork = nullptr;
@@ -323,9 +338,11 @@ int batch_tables(BatchGroup& g, int coupling, const jw::Alpha& al, const double*
 // one step of group g on its own stream, noise step from the device counter
 hipError_t batch_group_step(jw_batch* b, BatchGroup& g, int coupling, double step, const double* sa_pos,
                             const double* sa_k, int reunit, int count, void* U, int first_noise, int next_noise) {
-    jw::NoiseSrc ns{nullptr, b->seed, b->traj0 + (uint32_t)g.first, 0u, JW_TAG_LANGEVIN, b->d_step};
+    jw::NoiseSrc ns{nullptr, b->seed, b->traj0 + (uint32_t)g.first, 0u, b->kspace ? JW_TAG_KSPACE : JW_TAG_LANGEVIN,
+                    b->d_step};
     const jw::StepArgs a{ns,      sa_pos,      sa_k,  coupling == JW_SQRT_COUPLING_CONSTANT,
-                         coupling == JW_CORRELATED_NOISE_COUPLING, sqrt(step), reunit, count, first_noise, next_noise};
+                         coupling == JW_CORRELATED_NOISE_COUPLING, sqrt(step), reunit, count, first_noise, next_noise,
+                         b->kspace};
     if (b->fp32)
         return jw::step_fused_f32(g.ctx, (jw::cplxf*)U, (jw::cplxf*)g.Xi, (jw::cplxf*)g.Vv, (jw::cplxf*)g.ZA,
                                   (jw::cplxf*)g.ZB, a);
@@ -777,6 +794,13 @@ int jw_correlate_noise(jw_ctx* c, jw_field* f, const jw_alpha* a) {
     return JW_OK;
 }
 
+int jw_set_noise_space(jw_ctx* c, int space) {
+    if (!c) return fail(JW_ERR_ARG, "null context");
+    if (space != JW_NOISE_POSITION && space != JW_NOISE_MOMENTUM) return fail(JW_ERR_ARG, "noise space %d", space);
+    c->noise_kspace = space == JW_NOISE_MOMENTUM;
+    return JW_OK;
+}
+
 int jw_set_mv_zero_mode(jw_ctx* c, int mode) {
     if (!c) return fail(JW_ERR_ARG, "null context");
     if (mode != 0 && mode != 1) return fail(JW_ERR_ARG, "zero mode %d: 0 (reference) or 1 (projected out)", mode);
@@ -860,6 +884,7 @@ int jw_evolve(jw_ctx* c, jw_field* U, int coupling, int kernel, double step, con
     if (nsteps < 0) return fail(JW_ERR_ARG, "nsteps < 0");
     if (c->inj_per_site && nsteps > 1) return fail(JW_ERR_STATE, "an injection covers one step; nsteps = %d", nsteps);
     if (kernel == JW_LINEAR_KERNEL) {
+        if (c->noise_kspace) return fail(JW_ERR_UNSUPPORTED, "momentum-space noise runs the SIN kernel");
         for (int s = 0; s < nsteps; s++)
             if (int e = evolve_9comp(c, U, coupling, kernel, step, a)) return e;
         return JW_OK;
@@ -924,6 +949,7 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
     }
     const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
     const bool fused = jw::fused_supported(c->N) && !c->unfused;
+    if (int e = check_kspace(c, coupling, fused)) return e;
     const int saved_layout = c->layout;
     if (c->layout < 0) c->layout = default_layout(c->N, corr, 1);
     int rc = JW_OK;
@@ -934,15 +960,16 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
     if (rc == JW_OK && jw::u_convert(c, U32, U->d, 1) != hipSuccess) rc = fail(JW_ERR_HIP, "U conversion failed");
     for (int it = 0; rc == JW_OK && it < nsteps; it++) {
         bool ok;
-        jw::NoiseSrc ns = take_noise(c, 16, JW_TAG_LANGEVIN, &ok);
+        jw::NoiseSrc ns = take_noise(c, 16, c->noise_kspace ? JW_TAG_KSPACE : JW_TAG_LANGEVIN, &ok);
         if (!ok) {
             rc = fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
             break;
         }
         // projection every reunit_every steps and after the last one
         const int reunit = reunit_every > 0 && ((it + 1) % reunit_every == 0 || it == nsteps - 1);
-        const jw::StepArgs sa{ns,   sa_pos,     sa_k,   coupling == JW_SQRT_COUPLING_CONSTANT, corr, sqrt(step), reunit, 1,
-                              it == 0, it < nsteps - 1};
+        const jw::StepArgs sa{ns,      sa_pos,        sa_k,           coupling == JW_SQRT_COUPLING_CONSTANT,
+                              corr,    sqrt(step),    reunit,         1,
+                              it == 0, it < nsteps - 1, c->noise_kspace};
         const hipError_t he = fused ? jw::step_fused_f32(c, U32, Xi, Vv, ZA, ZB, sa)
                                     : jw::step_unfused_f32(c, U32, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k,
                                                            coupling == JW_SQRT_COUPLING_CONSTANT ? 0 : 1, corr,
@@ -1227,6 +1254,19 @@ int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step) {
     return JW_OK;
 }
 
+int jw_batch_set_noise_space(jw_batch* b, int space) {
+    if (!b) return fail(JW_ERR_ARG, "null batch");
+    if (space != JW_NOISE_POSITION && space != JW_NOISE_MOMENTUM) return fail(JW_ERR_ARG, "noise space %d", space);
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    b->kspace = space == JW_NOISE_MOMENTUM;
+    if (b->graph) {      // baked into the graph
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->have_key = false;
+    }
+    return JW_OK;
+}
+
 int jw_batch_set_layout(jw_batch* b, int group, int layout) {
     if (!b || group < 0 || group >= (int)b->g.size()) return fail(JW_ERR_ARG, "bad batch group");
     if (layout != -1 && layout != 0 && layout != 2 && layout != 4 && layout != 8 && layout != 16)
@@ -1255,6 +1295,8 @@ int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw
     if (coupling < 0 || coupling > 4 || kernel != JW_SIN_KERNEL)
         return fail(kernel == JW_LINEAR_KERNEL ? JW_ERR_UNSUPPORTED : JW_ERR_ARG, "batches run the SIN kernel");
     if (!(step >= 0) || nsteps < 0 || reunit_every < 0) return fail(JW_ERR_ARG, "bad step/nsteps/reunit_every");
+    if (b->kspace && coupling == JW_NOISE_COUPLING_CONSTANT)
+        return fail(JW_ERR_UNSUPPORTED, "momentum-space noise: NOISE coupling scales the noise per site");
     if (nsteps == 0) return JW_OK;
     if (!b->fp32) reunit_every = 0;      // the FP64 form is not projected (as jw_evolve)
     HIP_CHECK(hipSetDevice(b->device));

@@ -333,6 +333,51 @@ __device__ __forceinline__ void box_muller_pair_f(uint32_t site, uint32_t step, 
     z1 = -rr * s;
 }
 
+// ---------------------------------------------------------------------------
+// Momentum-space noise.  The spectrum of a pair of independent white fields,
+// F[n_a + i n_b](k), is itself white circular complex Gaussian: real and
+// imaginary parts N(0, V), independent across k.  So the xi spectra can be
+// drawn per mode instead of per site and transformed: Philox call
+// {k = kx N + ky, step, tag, p} (key {seed, traj}) gives four 32-bit uniforms
+// u_j = (w_j + 1/2) 2^-32 and two Box-Muller pairs
+//   (z0, z1) = sqrt(-2 ln u0) (cos 2 pi u1, sin 2 pi u1),  (z2, z3) from u2, u3,
+// with X^_p(k) = sqrt(V) (z0 + i z1), Y^_p(k) = sqrt(V) (z2 + i z3) for the x-
+// and y-direction plane pairs p.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void box_muller32(uint32_t w0, uint32_t w1, double& z0, double& z1) {
+    const double u0 = ((double)w0 + 0.5) * 0x1.0p-32;
+    const double u1 = ((double)w1 + 0.5) * 0x1.0p-32;
+    const double rr = sqrt(-2.0 * log(u0));
+    double s, co;
+    sincospi(2.0 * u1, &s, &co);
+    z0 = rr * co;
+    z1 = rr * s;
+}
+// FP32 from the same words, as box_muller_pair_f: log u0 below 1/2, above it
+// log1p(-t) with t = 1 - u0 taken from the integer; the angle pi x with
+// x = 2 u1 - 1 in (-1, 1)
+__device__ __forceinline__ void box_muller32(uint32_t w0, uint32_t w1, float& z0, float& z1) {
+    const bool lo = w0 < 0x80000000u;
+    const float t = (float)((double)(0xFFFFFFFFu - w0) + 0.5) * 0x1.0p-32f;
+    const float a = lo ? (float)((double)w0 + 0.5) * 0x1.0p-32f : 1.0f - t;
+    float lu = logf(a);
+    if (!lo) lu = a == 1.0f ? -t : lu * (t * __builtin_amdgcn_rcpf(1.0f - a));
+    const float rr = sqrtf(-2.0f * lu);
+    const int64_t d = 2 * (int64_t)w1 + 1 - (1ll << 32);            // x = d 2^-32
+    float s, co;
+    sincospif((float)(double)d * 0x1.0p-32f, &s, &co);
+    z0 = -rr * co;
+    z1 = -rr * s;
+}
+template <typename T>
+__device__ __forceinline__ void kspace_normals4(uint32_t k, uint32_t step, uint32_t tag, uint32_t p, uint32_t seed,
+                                                uint32_t traj, T z[4]) {
+    uint32_t c[4] = {k, step, tag, p};
+    philox4x32_10(c, seed, traj);
+    box_muller32(c[0], c[1], z[0], z[1]);
+    box_muller32(c[2], c[3], z[2], z[3]);
+}
+
 // Projection of a near-unitary 3x3 matrix onto SU(3): Gram-Schmidt on rows 0
 // and 1, row 2 = conj(row0 x row1) (unit determinant).  The periodic
 // reunitarisation of the mixed-precision form.

@@ -393,8 +393,12 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
 // ---------------------------------------------------------------------------
 // column pass: Z[p] = F_col^-1[ i (kx X^[p] + ky Y^[p]) / V ], X^ = F_col X
 //   kx = -2 pi c pbarX / phat2 (c = sa(k) for SQRT, else 1), zero at k = 0.
-//   CORR: X^, Y^ are first multiplied by sa(k) (per-mode correlated noise) and
-//   F_col^-1[sa X^ / V] is written back over X (likewise Y).
+//   CORR: X^, Y^ are first multiplied by sa(k) (per-mode correlated noise;
+//   by 1 when noise_scale is 0) and F_col^-1[sa X^ / V] is written back over X
+//   (likewise Y).
+//   GEN (with CORR): X^, Y^ are not transformed from X, Y but drawn per mode
+//   (kspace_normals4, plane pair p = blockIdx.y, noise key from ns); X and Y
+//   then receive the row spectra of the position-space noise.
 // X, Y: row spectra of plane p = blockIdx.y; strips of W columns.  PI: X is
 // the pair-interleaved array (layout 2) holding both directions, Y unused.
 // LDS row x holds [X(x, c0..c0+W-1), Y(x, c0..c0+W-1)]: 2W sequences of length N
@@ -403,13 +407,14 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
 // MF = 1, white noise (FP32 tiles of 74 KB, two per CU): registers of two
 // workgroups asked for, as the FP32 butterfly version fits them by itself; the
 // correlated variant needs more than that without spilling and runs one per CU
-template <int N, int CORR, typename C, int PI = 0, int MF = 0>
+template <int N, int CORR, typename C, int PI = 0, int MF = 0, int GEN = 0>
 __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, C, CORR>::NT / 256 : 1)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
                                                           C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                           const double* __restrict__ q4,
                                                           const double* __restrict__ sb,
                                                           const double* __restrict__ sa,
-                                                          const C* __restrict__ tw) {
+                                                          const C* __restrict__ tw, NoiseSrc ns, int noise_scale) {
+    static_assert(!GEN || CORR, "drawn spectra are written back as position-space noise");
     typedef typename scalar_of<C>::type T;
     typedef ColE<N, C, CORR> CE;
     constexpr int W = CE::W, NT = CE::NT, PER = CE::PER, PAIRS = CE::PAIRS;
@@ -437,15 +442,18 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, 
             g = pofs + (size_t)x * N + col0 + (m % W);
         }
     };
+    if constexpr (!GEN) {
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        int x, s;
-        size_t g;
-        elem(tid + q * NT, x, s, g);
-        lds[lpos<S2, 1, 3>(x, s)] = (PI || s < W) ? X[g] : Y[g];
+        for (int q = 0; q < PER; q++) {
+            int x, s;
+            size_t g;
+            elem(tid + q * NT, x, s, g);
+            lds[lpos<S2, 1, 3>(x, s)] = (PI || s < W) ? X[g] : Y[g];
+        }
+        __syncthreads();
+        lds_fft<N, S2, S2, 1, NT, -1, 3, MF>(lds, tw_fft(twl), opaque_v(tid));
     }
-    __syncthreads();
-    lds_fft<N, S2, S2, 1, NT, -1, 3, MF>(lds, tw_fft(twl), opaque_v(tid));
+    const NoiseSrc nk = batch_noise(ns, V);
     // pair (x, c): X^ in slot c of LDS row x, Y^ in slot W + c
     C zreg[CORR ? PAIRS : 1];
     // FP32 correlated: indices recomputed per phase (opaque_v), since addresses
@@ -458,7 +466,16 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, 
         const int pr = tk + q * NT;
         const int kx = pr / W, c = pr % W, ky = col0 + c;
         const size_t k = (size_t)kx * N + ky;
-        C vx = lds[lpos<S2, 1, 3>(kx, c)], vy = lds[lpos<S2, 1, 3>(kx, W + c)];
+        C vx, vy;
+        if constexpr (GEN) {
+            T g[4];
+            kspace_normals4((uint32_t)k, nk.step, nk.tag, blockIdx.y, nk.seed, nk.traj, g);
+            vx = mkc<C>(g[0] * (T)N, g[1] * (T)N);          // sqrt(V) = N
+            vy = mkc<C>(g[2] * (T)N, g[3] * (T)N);
+        } else {
+            vx = lds[lpos<S2, 1, 3>(kx, c)];
+            vy = lds[lpos<S2, 1, 3>(kx, W + c)];
+        }
         double fx = 0.0, fy = 0.0;
         if (kx != 0 || ky != 0) {
             const double ph = q4[kx] + q4[ky];
@@ -467,9 +484,11 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, 
             fy = -2.0 * M_PI * cc * sb[ky] / ph * invV;
         }
         if (CORR) {
-            const T s = (T)sa[k];
-            vx = cscale(vx, s);
-            vy = cscale(vy, s);
+            if (noise_scale) {
+                const T s = (T)sa[k];
+                vx = cscale(vx, s);
+                vy = cscale(vy, s);
+            }
             lds[lpos<S2, 1, 3>(kx, c)] = cscale(vx, (T)invV);
             lds[lpos<S2, 1, 3>(kx, W + c)] = cscale(vy, (T)invV);
         }
@@ -539,13 +558,15 @@ struct ColB {
                   "blocked column tiling");
 };
 
-template <int N, int BF, int CORR, typename C>
+template <int N, int BF, int CORR, typename C, int GEN = 0>
 __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void k_col_kmult_blk(C* __restrict__ Z, C* __restrict__ X,
                                                                      C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                                      const double* __restrict__ q4,
                                                                      const double* __restrict__ sb,
                                                                      const double* __restrict__ sa16,
-                                                                     const C* __restrict__ tw) {
+                                                                     const C* __restrict__ tw, NoiseSrc ns,
+                                                                     int noise_scale) {
+    static_assert(!GEN || CORR, "drawn spectra are written back as position-space noise");
     typedef typename scalar_of<C>::type T;
     constexpr int SEG = ColB<N, BF, C>::SEG, CPB = ColB<N, BF, C>::CPB, RB = ColB<N, BF, C>::RB;
     constexpr int NT = ColB<N, BF, C>::NT, PER = ColB<N, BF, C>::PER;
@@ -586,14 +607,20 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
     // one direction d (0: x, 1: y): forward column FFT of P, accumulate
     // i f_d P^ into acc for this thread's pairs (kx = pr mod N, column c = pr / N)
     // with f_d = -2 pi cc sb[k_d] / phat^2 / V, zero at k = 0; CORR: scale P^
-    // by sa and write F_col^-1[sa P^ / V] back over P
+    // by sa (noise_scale) and write F_col^-1[sa P^ / V] back over P.  GEN: P^
+    // drawn per mode instead; the x direction's draw also gives the y spectrum
+    // (held in yg until then).
+    const NoiseSrc nk = batch_noise(ns, V);
+    C yg[GEN ? PER : 1];
     auto direction = [&](C* __restrict__ P, C acc[PER], int d) {
         const double* q4d = opaque(q4);
         const double* sbd = opaque(sb);
         const double* sad = opaque(sa16);
-        load(P);
-        __syncthreads();
-        lds_fft<N, CPB, 1, PN, NT, -1, 4>(lds, tw_fft(twl), opaque_v(tid));
+        if constexpr (!GEN) {
+            load(P);
+            __syncthreads();
+            lds_fft<N, CPB, 1, PN, NT, -1, 4>(lds, tw_fft(twl), opaque_v(tid));
+        }
         const int t = opaque_v(tid);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
@@ -601,9 +628,21 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
             const int kx = pr % N, c = pr / N, ky = y0 + c;
             const int l = lpos<1, PN, 4>(kx, c);
             const size_t k16 = ((size_t)(kx >> 4) * N + ky) * 16 + (kx & 15);
-            C v = lds[l];
+            C v;
+            if constexpr (GEN) {
+                if (d == 0) {
+                    T g[4];
+                    kspace_normals4((uint32_t)((size_t)kx * N + ky), nk.step, nk.tag, blockIdx.y, nk.seed, nk.traj, g);
+                    v = mkc<C>(g[0] * (T)N, g[1] * (T)N);      // sqrt(V) = N
+                    yg[q] = mkc<C>(g[2] * (T)N, g[3] * (T)N);
+                } else {
+                    v = yg[q];
+                }
+            } else {
+                v = lds[l];
+            }
             if (CORR) {
-                v = cscale(v, (T)sad[k16]);
+                if (noise_scale) v = cscale(v, (T)sad[k16]);
                 lds[l] = cscale(v, (T)invV);
             }
             double f = 0.0;
@@ -754,40 +793,50 @@ static const C* twiddles(const jw_ctx* c) {
     else return c->twf;
 }
 
+// corr: the correlated-noise variant (xi~ written back); gen: the xi spectra
+// drawn per mode from ns (implies the write-back); noise_scale: the drawn or
+// transformed spectra are multiplied by sa (correlated coupling)
 template <int N, int BF, typename C>
 static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int sqrt_coupling, int corr,
-                           int nbatch) {
+                           int nbatch, const NoiseSrc& ns, int gen, int noise_scale) {
     const size_t V = c->V;
     const C* tw = twiddles<C>(c);
+#define JW_COL_ARGS Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw, ns, noise_scale
     if constexpr (BF == 0 || BF == 2) {
         constexpr int W = ColE<N, C>::W, PI = BF == 2;
+        const dim3 grid(N / W, 4, nbatch);
         // split-BF16 matrix-core radix-16 stages: FP32 storage, row-major layout
         if constexpr (sizeof(C) == sizeof(float2) && BF == 0) {
             if (c->fft_mfma) {
-                if (corr)
-                    hipLaunchKernelGGL((k_col_kmult<N, 1, C, 0, 1>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 1>::NT), 0,
-                                       c->stream, Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
+                if (gen)
+                    hipLaunchKernelGGL((k_col_kmult<N, 1, C, 0, 1, 1>), grid, dim3(ColE<N, C, 1>::NT), 0, c->stream,
+                                       JW_COL_ARGS);
+                else if (corr)
+                    hipLaunchKernelGGL((k_col_kmult<N, 1, C, 0, 1>), grid, dim3(ColE<N, C, 1>::NT), 0, c->stream,
+                                       JW_COL_ARGS);
                 else
-                    hipLaunchKernelGGL((k_col_kmult<N, 0, C, 0, 1>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 0>::NT), 0,
-                                       c->stream, Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
+                    hipLaunchKernelGGL((k_col_kmult<N, 0, C, 0, 1>), grid, dim3(ColE<N, C, 0>::NT), 0, c->stream,
+                                       JW_COL_ARGS);
                 return hipGetLastError();
             }
         }
-        if (corr)
-            hipLaunchKernelGGL((k_col_kmult<N, 1, C, PI>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 1>::NT), 0, c->stream,
-                               Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
+        if (gen)
+            hipLaunchKernelGGL((k_col_kmult<N, 1, C, PI, 0, 1>), grid, dim3(ColE<N, C, 1>::NT), 0, c->stream, JW_COL_ARGS);
+        else if (corr)
+            hipLaunchKernelGGL((k_col_kmult<N, 1, C, PI>), grid, dim3(ColE<N, C, 1>::NT), 0, c->stream, JW_COL_ARGS);
         else
-            hipLaunchKernelGGL((k_col_kmult<N, 0, C, PI>), dim3(N / W, 4, nbatch), dim3(ColE<N, C, 0>::NT), 0, c->stream,
-                               Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw);
+            hipLaunchKernelGGL((k_col_kmult<N, 0, C, PI>), grid, dim3(ColE<N, C, 0>::NT), 0, c->stream, JW_COL_ARGS);
     } else {
         constexpr int CPB = ColB<N, BF, C>::CPB, NT = ColB<N, BF, C>::NT;
-        if (corr)
-            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1, C>), dim3(N / CPB, 4, nbatch), dim3(NT), 0, c->stream, Z, X, Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
+        const dim3 grid(N / CPB, 4, nbatch);
+        if (gen)
+            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1, C, 1>), grid, dim3(NT), 0, c->stream, JW_COL_ARGS);
+        else if (corr)
+            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1, C>), grid, dim3(NT), 0, c->stream, JW_COL_ARGS);
         else
-            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 0, C>), dim3(N / CPB, 4, nbatch), dim3(NT), 0, c->stream, Z, X, Y, V,
-                               sqrt_coupling, c->q4, c->sb, sa_k, tw);
+            hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 0, C>), grid, dim3(NT), 0, c->stream, JW_COL_ARGS);
     }
+#undef JW_COL_ARGS
     return hipGetLastError();
 }
 
@@ -802,9 +851,12 @@ static hipError_t step_fused_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, cons
     // 7134 against 6670 / 6579); at 1024 and 2048 the separate pass is faster
     // (1916 / 1742 against 1746 / 1597; 471 / 426 against 463 / 420), and in
     // FP32 the separate pass, at two workgroups per CU, is faster everywhere.
+    // Momentum-space noise (a.kspace): no noise pass; the first column pass
+    // draws the spectra and writes the position-space noise's row spectra back
+    // for the correlated row pass.
     constexpr bool fuse_noise = sizeof(C) == sizeof(double2) && N <= 512;
-    const int first_noise = fuse_noise ? a.first_noise : 1;
-    const int next_noise = fuse_noise ? a.next_noise : 0;
+    const int first_noise = a.kspace ? 0 : fuse_noise ? a.first_noise : 1;
+    const int next_noise = a.kspace ? 0 : fuse_noise ? a.next_noise : 0;
     if (first_noise) {
         KTimer t(c, kcls[0]);
         hipLaunchKernelGGL((k_row_noise_fwd<N, BF, C>), dim3(N, 1, a.nbatch), dim3(RowE<N>::NT), 0, c->stream, Xi, V, a.ns,
@@ -813,11 +865,13 @@ static hipError_t step_fused_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, cons
     }
     {
         KTimer t(c, kcls[1]);
-        if ((e = col_pass<N, BF>(c, ZA, Xi, Xi + 4 * V, a.sa_k, a.sqrt_coupling, a.corr, a.nbatch)) != hipSuccess) return e;
+        if ((e = col_pass<N, BF>(c, ZA, Xi, Xi + 4 * V, a.sa_k, a.sqrt_coupling, a.corr, a.nbatch, a.ns, a.kspace,
+                                 a.corr)) != hipSuccess)
+            return e;
     }
     {
         KTimer t(c, kcls[2]);
-        if (a.corr)
+        if (a.corr || a.kspace)
             hipLaunchKernelGGL((k_row_uxiu_corr<N, BF, C>), dim3(N, 1, a.nbatch), dim3(RowE<N>::NT), 0, c->stream, Vv, U, Xi, V,
                                tw);
         else
@@ -827,7 +881,8 @@ static hipError_t step_fused_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, cons
     }
     {
         KTimer t(c, kcls[1]);
-        if ((e = col_pass<N, BF>(c, ZB, Vv, Vv + 4 * V, a.sa_k, a.sqrt_coupling, 0, a.nbatch)) != hipSuccess) return e;
+        if ((e = col_pass<N, BF>(c, ZB, Vv, Vv + 4 * V, a.sa_k, a.sqrt_coupling, 0, a.nbatch, a.ns, 0, 0)) != hipSuccess)
+            return e;
     }
     {
         KTimer t(c, kcls[3]);

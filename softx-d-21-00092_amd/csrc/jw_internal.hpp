@@ -15,6 +15,7 @@
 // Philox counter tags of the two noise streams
 #define JW_TAG_LANGEVIN 0x4C414E47u   // 'LANG'
 #define JW_TAG_MV 0x4D565348u         // 'MVSH'
+#define JW_TAG_KSPACE 0x4B535043u     // 'KSPC': momentum-space Langevin noise
 
 namespace jw {
 
@@ -79,6 +80,9 @@ struct jw_ctx {
     // 1: solve_poisson sets the k = 0 mode to zero instead of multiplying it
     // by -g/mass^2 (jw_set_mv_zero_mode)
     int mv_zero_mode;
+    // 1: the fused step draws the Langevin noise per momentum mode
+    // (kspace_normals4) instead of per site (jw_set_noise_space)
+    int noise_kspace;
     // per-kernel timing (HIP events on ctx->stream)
     int timing;
     double kernel_ms[16];
@@ -136,6 +140,7 @@ struct StepArgs {
     int nbatch;              // trajectories per launch (grid z; 1 for a single lattice)
     int first_noise;         // generate this step's xi first (else the previous update did)
     int next_noise;          // the update pass also generates the next step's xi
+    int kspace;              // xi drawn per mode in the first column pass (no noise pass)
 };
 hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a);
 hipError_t step_fused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a);

@@ -18,6 +18,10 @@
 //                           reference; 0: they are read as doubles
 //   mv_zero_mode = 0        0: the reference's k = 0 factor -g/m^2 in solvePoisson;
 //                           1: each MV sheet's k = 0 mode is set to zero
+//   noise_space = position  position: 16 Philox normals per site (the reference's
+//                           generate_gaussian); momentum: the noise spectra drawn
+//                           per mode (same distribution, no noise pass; fused
+//                           and batched engines, couplings other than NOISE)
 //   batch = 256:48,512:32   batched engine: size groups N:count (uneven sizes);
 //                           each group writes its own output file
 //   checkpoint_every = 0    steps between checkpoints of (U, noise state) per
@@ -81,6 +85,7 @@ class config {
     int reunit_every = 10;
     bool float_parse = true;
     int mv_zero_mode = 0;
+    bool noise_momentum = false;
     std::vector<std::pair<int, int>> batch;     // (N, count) size groups
     int checkpoint_every = 0;
     bool resume = false;
@@ -149,6 +154,14 @@ class config {
             } else if (!strcmp(key, "reunit_every")) reunit_every = atoi(val);
             else if (!strcmp(key, "float_parse")) float_parse = atoi(val) != 0;
             else if (!strcmp(key, "mv_zero_mode")) mv_zero_mode = atoi(val);
+            else if (!strcmp(key, "noise_space")) {
+                if (!strcmp(val, "momentum")) noise_momentum = true;
+                else if (!strcmp(val, "position")) noise_momentum = false;
+                else {
+                    fprintf(stderr, "config: noise_space %s: expected position or momentum\n", val);
+                    ok = 0;
+                }
+            }
             else if (!strcmp(key, "checkpoint_every")) checkpoint_every = atoi(val);
             else if (!strcmp(key, "resume")) resume = atoi(val) != 0;
             else if (!strcmp(key, "stop_after")) stop_after = atoi(val);
@@ -165,6 +178,10 @@ class config {
         }
         if (engine == ENGINE_REFERENCE && mixed) {
             fprintf(stderr, "config: the reference engine runs in FP64 only\n");
+            ok = 0;
+        }
+        if (engine == ENGINE_REFERENCE && noise_momentum) {
+            fprintf(stderr, "config: the reference engine draws position-space noise\n");
             ok = 0;
         }
         return ok;

@@ -219,6 +219,8 @@ class fftw2D {
         s.alpha = jw_alpha{cnfg->lambda_L, cnfg->mu0_over_lambda, 0.2, 9.0};
         jwhost::check(jw_noise_philox(s.ctx, cnfg->seed, 0, 0), "noise stream");
         jwhost::check(jw_set_mv_zero_mode(s.ctx, cnfg->mv_zero_mode), "MV zero mode");
+        jwhost::check(jw_set_noise_space(s.ctx, cnfg->noise_momentum ? JW_NOISE_MOMENTUM : JW_NOISE_POSITION),
+                      "noise space");
         return 1;
     }
     template <class T, int t>

@@ -136,6 +136,8 @@ static int run_batch_list(config* cnfg, mpi_class* mpi) {
     clock_gettime(CLOCK_MONOTONIC, &start);
     jw_batch* batch = nullptr;
     jwhost::check(jw_batch_create(&batch, 0, G, Ns.data(), counts.data(), cnfg->mixed ? 1 : 0), "batch");
+    jwhost::check(jw_batch_set_noise_space(batch, cnfg->noise_momentum ? JW_NOISE_MOMENTUM : JW_NOISE_POSITION),
+                  "noise space");
     const int Ny = cnfg->elementaryWilsonLines;
     int t = 0;
     for (int g = 0; g < G; g++)
@@ -290,6 +292,8 @@ int main(int argc, char* argv[]) {
             jw_batch* batch = nullptr;
             const int bN = Nx, bcount = cnfg->stat;
             jwhost::check(jw_batch_create(&batch, 0, 1, &bN, &bcount, cnfg->mixed ? 1 : 0), "batch");
+            jwhost::check(jw_batch_set_noise_space(batch, cnfg->noise_momentum ? JW_NOISE_MOMENTUM : JW_NOISE_POSITION),
+                          "noise space");
             for (int stat = 0; stat < cnfg->stat; stat++) {
                 printf("Gathering stat sample no. %i\n", stat);
                 mv_init(stat);

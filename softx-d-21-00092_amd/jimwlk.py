@@ -18,7 +18,8 @@ LIB_PATH = os.path.join(HERE, "libjimwlk_hip.so")
 SQRT_COUPLING_CONSTANT, NOISE_COUPLING_CONSTANT, HATTA_COUPLING_CONSTANT = 0, 1, 2
 FIXED_COUPLING, CORRELATED_NOISE_COUPLING = 3, 4
 LINEAR_KERNEL, SIN_KERNEL = 0, 1
-TAG_LANGEVIN, TAG_MV = 0x4C414E47, 0x4D565348
+TAG_LANGEVIN, TAG_MV, TAG_KSPACE = 0x4C414E47, 0x4D565348, 0x4B535043
+NOISE_POSITION, NOISE_MOMENTUM = 0, 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -94,6 +95,8 @@ def lib():
         L.jw_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_set_fused.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_set_mixed_fft.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_set_noise_space.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_batch_set_noise_space.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_calibration_copy.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_batch_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -320,6 +323,11 @@ class Context:
         DFT-matrix products on the matrix cores."""
         _check(lib().jw_set_mixed_fft(self.h, int(mode)))
 
+    def set_noise_space(self, space):
+        """NOISE_POSITION (default): 16 Philox normals per site; NOISE_MOMENTUM: the
+        noise spectra drawn per mode in the first column pass (same distribution)."""
+        _check(lib().jw_set_noise_space(self.h, int(space)))
+
     def kernel_times(self):
         out = {}
         for k in range(13):
@@ -372,6 +380,9 @@ class Batch:
 
     def set_layout(self, group, layout):
         _check(lib().jw_batch_set_layout(self.h, group, layout))
+
+    def set_noise_space(self, space):
+        _check(lib().jw_batch_set_noise_space(self.h, int(space)))
 
     def layout(self, group):
         return lib().jw_batch_layout(self.h, group)

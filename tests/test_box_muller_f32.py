@@ -44,3 +44,29 @@ def test_naive_log_loses_small_radii():
     naive = np.sqrt(F(-2) * np.log(u0.astype(F))).astype(np.float64)
     assert np.abs(radius_f32(m0) - ref).max() <= 4 * 2.0 ** -24
     assert np.abs(naive - ref).max() > 1e-5
+
+
+def radius32_f32(w0):
+    """box_muller32's FP32 radius (momentum-space noise): the same method on one
+    32-bit word, u0 = (w0 + 1/2) 2^-32."""
+    lo = w0 < (1 << 31)
+    t = (((0xFFFFFFFF - w0).astype(np.float64) + 0.5).astype(F)) * F(2.0 ** -32)
+    a = np.where(lo, ((w0.astype(np.float64) + 0.5).astype(F)) * F(2.0 ** -32), F(1) - t)
+    lu = np.log(a).astype(F)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        hi = np.where(a == F(1), -t, (lu * (t * (F(1) / (F(1) - a)).astype(F))).astype(F))
+    lu = np.where(lo, lu, hi).astype(F)
+    return np.sqrt(F(-2) * lu).astype(F)
+
+
+def test_radius32_within_few_ulps():
+    rng = np.random.default_rng(6)
+    w0 = np.concatenate([
+        rng.integers(0, 1 << 32, 200_000, dtype=np.uint64),
+        (0xFFFFFFFF - rng.integers(0, 1 << 12, 20_000, dtype=np.uint64)).astype(np.uint64),
+        ((1 << 31) + rng.integers(-1000, 1000, 2000)).astype(np.uint64),
+        np.array([0xFFFFFFFF, 1 << 31, (1 << 31) - 1, 0, 1], dtype=np.uint64)])
+    u0 = (w0.astype(np.float64) + 0.5) * 2.0 ** -32
+    ref = np.sqrt(-2.0 * np.log(u0))
+    err = np.abs(radius32_f32(w0).astype(np.float64) - ref) / np.maximum(ref, 1.0)
+    assert err.max() <= 4 * 2.0 ** -24, err.max() / 2.0 ** -24

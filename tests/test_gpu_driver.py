@@ -39,9 +39,11 @@ def read_out(path):
     return np.array([[float(v) for v in r] for r in rows])
 
 
-def oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, coupling, seed=11, trajs=None, position=False):
+def oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, coupling, seed=11, trajs=None, position=False,
+                  kspace=False):
     """main_optimized.cpp:159-355 through the oracle with the driver's Philox stream
-    (stat sample s has the noise key {seed, trajs[s]}, trajs[s] = s by default)."""
+    (stat sample s has the noise key {seed, trajs[s]}, trajs[s] = s by default);
+    kspace: the position-space normals of the momentum-space stream."""
     mu = float(np.float32(30.72)) / N
     m = float(np.float32(mass))
     ds = float(np.float32(step))
@@ -53,7 +55,8 @@ def oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, coupling, seed=
         z = np.concatenate([orc.philox_normals(N, seed, s, k, jw.TAG_MV, 8) for k in range(Ny)])
         U = orc.mv_init(N, z, Ny, 1.0, mu, m)        # Poisson mass m g^2 mu inside
         for l in range(steps):
-            zl = orc.philox_normals(N, seed, s, Ny + l, jw.TAG_LANGEVIN, 16)
+            zl = orc.kspace_normals(N, seed, s, Ny + l, jw.TAG_KSPACE) if kspace else \
+                orc.philox_normals(N, seed, s, Ny + l, jw.TAG_LANGEVIN, 16)
             if position:           # main_optimized.cpp:243-300, POSITION_EVOLUTION
                 xx, xy = orc.generate_gaussian(N, zl)
                 if coupling == 1:
@@ -117,6 +120,35 @@ def test_driver_batched_and_mixed(jw, orc, tmp_path, engine, precision, rtol):
     assert got.shape == ref.shape and np.array_equal(got[:, :3], ref[:, :3])
     np.testing.assert_allclose(got[:, 3], ref[:, 3], rtol=0, atol=5e-7)     # printed with %f
     np.testing.assert_allclose(got[:, 4:], ref[:, 4:], rtol=rtol, atol=1e-12)
+
+
+@pytest.mark.parametrize("engine,precision,rtol", [("fused", "fp64", 2e-6), ("batched", "fp64", 2e-6),
+                                                   ("fused", "mixed", 1e-4)])
+def test_driver_momentum_noise(jw, orc, tmp_path, engine, precision, rtol):
+    """noise_space = momentum: the fused and batched engines draw the noise spectra
+    per mode; the oracle steps on the position-space normals of the same stream."""
+    assert os.path.exists(DRIVER), "driver not built"
+    N, stat, mass, Ny, steps, meas, step = 64, 2, 2.0, 6, 8, 4, 0.0004
+    cfg = tmp_path / "cfg"
+    write_cfg(cfg, N, stat, mass, Ny, steps, meas, step, "FIXED_COUPLING", engine, precision=precision,
+              extra="noise_space = momentum\n")
+    r = subprocess.run([DRIVER, "1", "1", str(cfg)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = read_out(tmp_path / f"out_{engine}_{precision}_{N}_{N}_mpi1_r0.dat")
+    ref = oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, 3, kspace=True)
+    pos = oracle_output(orc, jw, N, stat, mass, Ny, steps, meas, step, 3)
+    assert got.shape == ref.shape and np.array_equal(got[:, :3], ref[:, :3])
+    np.testing.assert_allclose(got[:, 3], ref[:, 3], rtol=0, atol=5e-7)     # printed with %f
+    np.testing.assert_allclose(got[:, 4:], ref[:, 4:], rtol=rtol, atol=1e-12)
+    assert not np.allclose(pos[:, 4:], ref[:, 4:], rtol=10 * rtol, atol=1e-12)     # another noise
+
+
+def test_driver_rejects_momentum_noise_in_reference_engine(tmp_path):
+    assert os.path.exists(DRIVER), "driver not built"
+    cfg = tmp_path / "cfg"
+    write_cfg(cfg, 64, 1, 2.0, 2, 2, 1, 0.0004, "FIXED_COUPLING", "reference", extra="noise_space = momentum\n")
+    r = subprocess.run([DRIVER, "1", "1", str(cfg)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
 
 
 def test_driver_measurement_schedule_uneven(jw, orc, tmp_path):

@@ -431,6 +431,67 @@ def test_philox_normals_statistics(orc):
     assert abs(np.corrcoef(z, z2)[0, 1]) < 0.02
 
 
+TAG_KSPACE = 0x4B535043
+
+
+def test_kspace_noise_spectrum_round_trip(orc):
+    """The position-space normals of the momentum-space stream transform back
+    (numpy's forward FFT, e^{-2 pi i k.x/N}, unnormalised) to N times the drawn
+    spectra, plane pair by plane pair and direction by direction: pins the sign
+    of the exponent, the 1/N normalisation and the draw-order slots."""
+    N = 32
+    z, spec = orc.kspace_normals(N, 5, 1, 3, TAG_KSPACE, spectra=True)
+    z = z.reshape(N * N, 16)
+    for p in range(4):
+        wx = (z[:, 4 * p] + 1j * z[:, 4 * p + 2]).reshape(N, N)
+        wy = (z[:, 4 * p + 1] + 1j * z[:, 4 * p + 3]).reshape(N, N)
+        px = (spec[:, p, 0] + 1j * spec[:, p, 1]).reshape(N, N)
+        py = (spec[:, p, 2] + 1j * spec[:, p, 3]).reshape(N, N)
+        assert np.abs(np.fft.fft2(wx) - N * px).max() < 1e-12 * N
+        assert np.abs(np.fft.fft2(wy) - N * py).max() < 1e-12 * N
+
+
+def test_kspace_noise_words_are_philox(orc):
+    """Mode k, pair p: the four Philox words of counter {k, step, tag, p} give the
+    spectra through Box-Muller on 32-bit uniforms (checked on a few modes against
+    a numpy evaluation of the published transform)."""
+    N = 16
+    seed, traj, step = 11, 2, 9
+    _, spec = orc.kspace_normals(N, seed, traj, step, TAG_KSPACE, spectra=True)
+    for k in (0, 1, 17, 255):
+        for p in range(4):
+            w = orc.philox4x32_10([k, step, TAG_KSPACE, p], [seed, traj])
+            u = (np.array(w, dtype=np.float64) + 0.5) * 2.0 ** -32
+            r0, r1 = np.sqrt(-2 * np.log(u[0])), np.sqrt(-2 * np.log(u[2]))
+            want = [r0 * np.cos(2 * np.pi * u[1]), r0 * np.sin(2 * np.pi * u[1]),
+                    r1 * np.cos(2 * np.pi * u[3]), r1 * np.sin(2 * np.pi * u[3])]
+            assert np.allclose(spec[k, p], want, rtol=1e-13, atol=1e-13)
+
+
+def test_kspace_noise_is_white_unit_gaussian(orc):
+    """In distribution the momentum-space stream is the position-space one: 16
+    independent unit normals per site, uncorrelated between sites and steps.
+    Bounds are about 5 standard errors of each estimate."""
+    N = 64
+    V = N * N
+    z = orc.kspace_normals(N, 7, 0, 0, TAG_KSPACE).reshape(V, 16)
+    n = z.size
+    assert abs(z.mean()) < 5 / np.sqrt(n)
+    assert abs(z.var() - 1) < 5 * np.sqrt(2 / n)
+    # component covariance: identity (off-diagonal standard error 1/sqrt(V))
+    c = np.cov(z.T)
+    assert np.abs(np.diag(c) - 1).max() < 6 * np.sqrt(2 / V)
+    assert np.abs(c - np.diag(np.diag(c))).max() < 6 / np.sqrt(V)
+    # neighbours in x and y, and the next step: uncorrelated
+    f = z.reshape(N, N, 16)
+    for g in (np.roll(f, 1, 0), np.roll(f, 1, 1), orc.kspace_normals(N, 7, 0, 1, TAG_KSPACE).reshape(N, N, 16)):
+        assert abs(np.mean(f * g)) < 5 / np.sqrt(n)
+    # Gaussian tails: fourth moment 3, and the KS distance to N(0, 1)
+    assert abs(np.mean(z ** 4) - 3) < 5 * np.sqrt(96 / n)
+    from scipy import stats
+    assert stats.kstest(z.ravel(), "norm").pvalue > 1e-4
+
+
 # ---------------------------------------------------------------- POSITION_EVOLUTION
 @pytest.mark.parametrize("coupling", [0, 1])
 @pytest.mark.parametrize("kernel", [0, 1])
