@@ -13,12 +13,17 @@ Forms, each timed over the same K steps from the same start:
   * fp64  -- jw_evolve: FP64 throughout (the oracle-parity path);
   * mixed -- jw_evolve_mixed: FP32 fields, FFTs and products, FP64 exponentials, U
              projected onto SU(3) every 10 steps;
+  * fp64_k, mixed_k -- the same with momentum-space noise (jw_set_noise_space:
+             the noise spectra drawn per mode, the same distribution as the
+             per-site normals, no noise pass);
   * mixed+batched -- jw_batch_evolve with --batch N = 2048 trajectories in one
              batch (FP32, captured step graph, tuned layout).
-`value` is the mixed+batched form's trajectory-steps/s (whole job); the FP64
-form's steps/s, the ratio and S(r) after --sr-steps steps of the FP64 and
-mixed forms from the same MV start and Philox stream (max |S_mixed(r) -
-S_fp64(r)| over all lattice separations r) sit beside it.  `ensemble` times
+`value` is the mixed+batched form's trajectory-steps/s (whole job) with the noise
+of --noise-space (default momentum); the FP64 form's steps/s with the same
+noise, the ratio and S(r) after --sr-steps steps of the FP64 and mixed forms
+from the same MV start and Philox stream (max |S_mixed(r) - S_fp64(r)| over all
+lattice separations r) sit beside it, and every form's steps/s in
+`forms_steps_per_s`.  `ensemble` times
 the uneven batched ensemble 48 x 256^2 + 32 x 512^2 + 16 x 1024^2 (fixed
 coupling, Delta s = 1e-4) in both precisions, in site-updates/s.
 
@@ -92,10 +97,12 @@ def parse():
                    help="jw_set_fused mode: 1 default, 2/3/4 row-major/4-row/8-row blocked intermediates")
     p.add_argument("--forms", default="fp64,mixed,fp64_k,mixed_k",
                    help="forms to time (profiling runs may pick one); *_k: momentum-space noise")
-    p.add_argument("--noise-space", default="position", choices=["position", "momentum"],
+    p.add_argument("--noise-space", default="momentum", choices=["position", "momentum"],
                    help="noise of the headline forms, the batched form and the S(r) comparison")
     p.add_argument("--batch", type=int, default=2, help="N = 2048 trajectories of the batched form (0: skip)")
     p.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble")
+    p.add_argument("--large-steps", type=int, default=200,
+                   help="steps of the N = 8192 single-lattice configuration in both forms (0: skip)")
     return p.parse_args()
 
 
@@ -268,6 +275,71 @@ def main():
                   f"(layouts {lays})", file=sys.stderr, flush=True)
         ensemble["mixed_over_fp64"] = round(ensemble["mixed"]["steps_per_s"] / ensemble["fp64"]["steps_per_s"], 3)
 
+    # the large single lattice: N = 8192, MV start Ny = 100, g^2 mu a = 0.05,
+    # correlated coupling, Delta s = 5e-5; both forms from the same start and
+    # noise, timed over all their steps; S(r) and Q_s of the mixed form against FP64
+    large = None
+    if a.large_steps > 0:
+        NL = 8192
+        cl = jw.Context(NL, 0)
+        al_l = jw.alpha(lambda_L=LAMBDA_A * NL, mu0_over_lambda=math.exp(2 * math.pi / (9 * ALPHA_MAX)))
+        UL = cl.field()
+        cl.philox(11, 0, 0)
+        t0 = time.perf_counter()
+        cl.mv_init(UL, 100, 1.0, 0.05, 1e-4 / 0.05)
+        cl.sync()
+        t_mv = time.perf_counter() - t0
+        UL0 = UL.download()
+        UL.close()
+        cl.set_noise_space(jw.NOISE_MOMENTUM if ks else jw.NOISE_POSITION)
+        large = {"N": NL, "sites": NL * NL, "mv": {"Ny": 100, "g2mu_a": 0.05, "m_a": 1e-4, "ms": round(t_mv * 1e3, 1)},
+                 "coupling": "correlated", "step": 5e-5, "steps": a.large_steps, "noise_space": a.noise_space}
+        SL, QL = {}, {}
+        for name in ("fp64", "mixed"):
+            F = cl.field(9, UL0)
+            cl.philox(12, 0, 100)
+            cl.sync()
+            t0 = time.perf_counter()
+            if name == "mixed":
+                cl.evolve_mixed(F, coupling, jw.SIN_KERNEL, 5e-5, al_l, nsteps=a.large_steps, reunit_every=10)
+            else:
+                cl.evolve(F, coupling, jw.SIN_KERNEL, 5e-5, al_l, nsteps=a.large_steps)
+            cl.sync()
+            dt = time.perf_counter() - t0
+            SL[name] = cl.dipole_from_corr(cl.measure_corr(F))
+            r_, s_, c_ = jw.radial_bins(SL[name], NL, NL // 2)
+            QL[name] = jw.saturation_scale(r_, s_, c_)
+            F.close()
+            bpr = 4 if name == "mixed" else 8
+            comp = REALS_PER_SITE["kspace" if ks else kind] * bpr * NL * NL
+            large[name] = {"steps_per_s": round(a.large_steps / dt, 3),
+                           "alg_GBs": round(comp * a.large_steps / dt / 1e9, 1),
+                           "alg_frac": round(comp * a.large_steps / dt / (HBM_PEAK_GBS * 1e9), 4), "Qs": QL[name]}
+            print(f"bench: N={NL} {name}: {a.large_steps} steps in {dt:.2f} s -> {a.large_steps / dt:.2f} steps/s, "
+                  f"Q_s = {QL[name]:.5f}", file=sys.stderr, flush=True)
+        # the matrix-core column stages: a short timed run
+        F = cl.field(9, UL0)
+        cl.philox(12, 0, 100)
+        cl.set_mixed_fft(1)
+        cl.evolve_mixed(F, coupling, jw.SIN_KERNEL, 5e-5, al_l, nsteps=2, reunit_every=10)
+        cl.sync()
+        t0 = time.perf_counter()
+        nm = min(20, a.large_steps)
+        cl.evolve_mixed(F, coupling, jw.SIN_KERNEL, 5e-5, al_l, nsteps=nm, reunit_every=10)
+        cl.sync()
+        large["mixed_mfma_steps_per_s"] = round(nm / (time.perf_counter() - t0), 3)
+        cl.set_mixed_fft(0)
+        F.close()
+        large["sr_err_vs_fp64"] = float(np.abs(SL["fp64"] - SL["mixed"]).max())
+        large["Qs_rel_err_vs_fp64"] = abs(QL["mixed"] - QL["fp64"]) / QL["fp64"] if QL["fp64"] else None
+        large["bytes_per_step_note"] = ("alg_GBs: the compulsory bytes of the fused step (%d reals per site); the "
+                                        "large-lattice step moves %d reals per site (row transforms and site "
+                                        "passes apart)" % (REALS_PER_SITE["kspace" if ks else kind],
+                                                           246 if ks else 294))
+        print(f"bench: N={NL} S(r) mixed vs fp64 {large['sr_err_vs_fp64']:.3e}, Q_s rel err "
+              f"{large['Qs_rel_err_vs_fp64']}", file=sys.stderr, flush=True)
+        cl.close()
+
     # measured traffic of the dominant kernels, when a profile summary is present
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -369,6 +441,7 @@ def main():
            "mixed_over_fp64": round(res[fmix]["steps_per_s"] / res[f64]["steps_per_s"], 3)
            if fmix in res and f64 in res else None,
            "ensemble": ensemble,
+           "large": large,
            "sr_err_vs_fp64": sr,
            "roofline": h["roofline"], "step_roofline": h["step_roofline"], "kernels": h["kernels"],
            "roofline_fp64": res[f64]["roofline"] if f64 in res else None,

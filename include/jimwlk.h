@@ -179,7 +179,12 @@ int jw_reduce(jw_ctx* ctx, const jw_field* corr, jw_field* sum, jw_field* err);
  * the generator basis (8 real planes per su(3) field, two per complex FFT).
  * Noise: the pending injection (nsteps must be 1) or the Philox stream (its
  * step counter advances by one per step).  step = Delta s as the caller wants
- * it (the reference parses it through float, config.h:92-93). */
+ * it (the reference parses it through float, config.h:92-93).  The SIN kernel
+ * runs the fused passes for 64 <= N <= 2048; at N = 4096 and 8192 the fused
+ * column passes (forward column FFT, kernel multiply, inverse column FFT per
+ * strip) with the row transforms and the site algebra as separate passes; the
+ * unfused kernel sequence otherwise (and with jw_set_fused(ctx, 0)).  All of
+ * them compute the same step. */
 int jw_evolve(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
               int nsteps);
 
@@ -188,12 +193,13 @@ int jw_evolve(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, c
  * main_optimized.cpp:234-311): U and every field and FFT in FP32, the SU(3)
  * exponentials and the projection of U onto SU(3) in FP64; the products
  * around the exponentials and U xi U^dagger are FP32 in the fused passes
- * (64 <= N <= 2048) and FP64 in the unfused ones.  U is
+ * (64 <= N <= 2048) and FP64 in the others.  U is
  * converted to FP32 on entry and back on return, and projected onto SU(3)
  * every reunit_every steps and after the last step (reunit_every = 0: never).
  * The noise is the same Philox stream (normals computed in FP32 from the same
  * words) or the pending injection.  SIN kernel; the fused passes for
- * 64 <= N <= 2048, the unfused ones otherwise. */
+ * 64 <= N <= 2048, at N = 4096 and 8192 the fused column passes with separate
+ * row transforms and site passes, the unfused ones otherwise. */
 int jw_evolve_mixed(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, const jw_alpha* alpha,
                     int nsteps, int reunit_every);
 
@@ -209,7 +215,7 @@ int jw_evolve_mixed(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double s
  * the x- and y-direction spectra of plane pair p (generators 2p, 2p+1) are
  * N (z0 + i z1) and N (z2 + i z3).  The position-space normals are then
  * n_2p + i n_2p+1 = N^-1 sum_k e^{+2 pi i k.x/N} (z0 + i z1)_k and likewise m
- * from (z2, z3).  Needs the fused step (64 <= N <= 2048), the SIN kernel and a
+ * from (z2, z3).  Needs the fused passes (64 <= N <= 8192), the SIN kernel and a
  * coupling other than NOISE_COUPLING_CONSTANT (which scales the noise per
  * site): JW_ERR_UNSUPPORTED otherwise, JW_ERR_STATE with a pending injection.
  * No counterpart in the reference. */
@@ -303,7 +309,10 @@ int jw_saturation_scale(int nbins, const double* rbar, const double* Sbin, const
  * directions of each plane pair interleaved site by site, so that a column
  * strip of both is one row segment; 7: row-major planes with the column pass
  * taking one plane at a time over 32-byte column strips (two workgroups per CU
- * in FP64) (2..7 for measurement). */
+ * in FP64); 8: one plane at a time over 16-byte strips (N = 4096 and 8192
+ * only).  At N = 4096 modes 0, 1, 2 and 8 exist, at N = 8192 modes 0, 1 and 8
+ * (the column strips that fit LDS); others are JW_ERR_UNSUPPORTED there
+ * (2..8 for measurement). */
 int jw_set_fused(jw_ctx* ctx, int mode);
 /* The column transforms of the mixed-precision form (jw_evolve_mixed, fused
  * passes, row-major layout).  mode 0 (default): FP32 radix-8/4/2 butterflies;

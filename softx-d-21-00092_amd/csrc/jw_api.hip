@@ -95,13 +95,13 @@ jw::NoiseSrc take_noise(jw_ctx* c, int per_site, uint32_t tag, bool* ok) {
     return ns;
 }
 
-// momentum-space noise runs on the fused passes (64 <= N <= 2048) with a
+// momentum-space noise runs on the fused passes (64 <= N <= 8192) with a
 // coupling that does not scale the noise per site, and replaces the Philox
 // draws (an injection is position-space noise)
 int check_kspace(const jw_ctx* c, int coupling, bool fused) {
     if (!c->noise_kspace) return JW_OK;
     if (!fused)
-        return fail(JW_ERR_UNSUPPORTED, "momentum-space noise needs the fused step (64 <= N <= 2048, fused mode on)");
+        return fail(JW_ERR_UNSUPPORTED, "momentum-space noise needs the fused passes (64 <= N <= 8192, fused mode on)");
     if (coupling == JW_NOISE_COUPLING_CONSTANT)
         return fail(JW_ERR_UNSUPPORTED, "momentum-space noise: NOISE coupling scales the noise per site");
     if (c->inj_per_site) return fail(JW_ERR_STATE, "momentum-space noise: injected normals are position-space noise");
@@ -120,27 +120,30 @@ int check_field(const jw_field* f, const jw_ctx* c, int t, const char* name) {
 // row-major 382 / 568, 418 / 653; 4-row blocks 363 / 474, 415 / 536; 8-row
 // blocks 345 / 507, 407 / 559.  Row-major everywhere.
 // the column passes of layouts 1, 4 and 8 read the 16-row-blocked sqrt(alpha) table
-bool uses_sa16(int layout) { return layout == 1 || layout == 4 || layout == 8 || layout == 16; }
+bool uses_sa16(int layout) { return layout == 1 || layout == 4 || layout == 8 || layout >= 16; }
 
+// N = 8192: 16-byte strips (the only ones that fit LDS); N = 4096: 16-byte
+// strips in FP32 (measured 143 against 125 steps/s for the X + Y column tiles,
+// correlated, momentum-space noise), the X + Y tiles in FP64 (89 against 85);
+// otherwise row-major X + Y column tiles
 int default_layout(int N, int corr, int fp32) {
-    (void)N;
     (void)corr;
-    (void)fp32;
-    return 0;
+    return N == 8192 || (N == 4096 && fp32) ? 32 : 0;
 }
 
 // nsteps generator-basis steps: the fused passes or the unfused kernel sequence
-int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, jw::cplx* Xi, jw::cplx* Vv, jw::cplx* ZA,
-                           jw::cplx* ZB, const double* sa_pos, const double* sa_k, int kc, int corr, double s,
-                           double ss) {
+int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, bool large, jw::cplx* Xi, jw::cplx* Vv,
+                           jw::cplx* ZA, jw::cplx* ZB, const double* sa_pos, const double* sa_k, int kc, int corr,
+                           double s, double ss) {
     const size_t V = c->V;
     for (int it = 0; it < nsteps; it++) {
         bool ok;
         jw::NoiseSrc ns = take_noise(c, 16, c->noise_kspace ? JW_TAG_KSPACE : JW_TAG_LANGEVIN, &ok);
         if (!ok) return fail(JW_ERR_STATE, "pending injection has %d normals per site, need 16", c->inj_per_site);
-        if (fused) {
+        if (fused || large) {
             const jw::StepArgs sa{ns, sa_pos, sa_k, kc == 0, corr, ss, 0, 1, it == 0, it < nsteps - 1, c->noise_kspace};
-            HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, sa));
+            if (fused) HIP_CHECK(jw::step_fused(c, U->d, Xi, Vv, ZA, ZB, sa));
+            else HIP_CHECK(jw::step_large(c, U->d, Xi, Vv, ZA, ZB, sa));
             continue;
         }
         HIP_CHECK(jw::step_unfused(c, U->d, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc, corr, ss));
@@ -182,10 +185,11 @@ int evolve_sin(jw_ctx* c, jw_field* U, int coupling, double step, const jw_alpha
     const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
     const double s = 1.0 / c->N, ss = sqrt(step);
     const bool fused = jw::fused_supported(c->N) && !c->unfused;
-    if (int e = check_kspace(c, coupling, fused)) return e;
+    const bool large = jw::large_supported(c->N) && !c->unfused;
+    if (int e = check_kspace(c, coupling, fused || large)) return e;
     const int saved_layout = c->layout;
     if (c->layout < 0) c->layout = default_layout(c->N, corr, 0);
-    if (fused && sa_k && uses_sa16(c->layout)) {
+    if ((fused || large) && sa_k && uses_sa16(c->layout)) {
         if (tabs) {
             const hipError_t he = jw::block16(c, tabs + V, tabs);
             if (he != hipSuccess) {
@@ -201,7 +205,7 @@ int evolve_sin(jw_ctx* c, jw_field* U, int coupling, double step, const jw_alpha
             sa_k = c->sa16;
         }
     }
-    const int rc = evolve_generator_basis(c, U, nsteps, fused, Xi, Vv, ZA, ZB, sa_pos, sa_k, kc, corr, s, ss);
+    const int rc = evolve_generator_basis(c, U, nsteps, fused, large, Xi, Vv, ZA, ZB, sa_pos, sa_k, kc, corr, s, ss);
     c->layout = saved_layout;
     return rc;
 }
@@ -949,11 +953,12 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
     }
     const int corr = coupling == JW_CORRELATED_NOISE_COUPLING;
     const bool fused = jw::fused_supported(c->N) && !c->unfused;
-    if (int e = check_kspace(c, coupling, fused)) return e;
+    const bool large = jw::large_supported(c->N) && !c->unfused;
+    if (int e = check_kspace(c, coupling, fused || large)) return e;
     const int saved_layout = c->layout;
     if (c->layout < 0) c->layout = default_layout(c->N, corr, 1);
     int rc = JW_OK;
-    if (fused && sa_k && uses_sa16(c->layout)) {
+    if ((fused || large) && sa_k && uses_sa16(c->layout)) {
         rc = ensure_sa16(c);
         sa_k = c->sa16;
     }
@@ -971,6 +976,7 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
                               corr,    sqrt(step),    reunit,         1,
                               it == 0, it < nsteps - 1, c->noise_kspace};
         const hipError_t he = fused ? jw::step_fused_f32(c, U32, Xi, Vv, ZA, ZB, sa)
+                              : large ? jw::step_large_f32(c, U32, Xi, Vv, ZA, ZB, sa)
                                     : jw::step_unfused_f32(c, U32, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k,
                                                            coupling == JW_SQRT_COUPLING_CONSTANT ? 0 : 1, corr,
                                                            sqrt(step), reunit);
@@ -1086,8 +1092,13 @@ static const char* kNames[] = {"fft_rows", "fft_cols", "gen_noise", "kernel_mult
 
 int jw_set_fused(jw_ctx* c, int mode) {
     if (!c) return fail(JW_ERR_ARG, "null context");
-    static const int layouts[8] = {0, JW_DEFAULT_LAYOUT, 0, 4, 8, 1, 2, 16};
-    if (mode < 0 || mode > 7) return fail(JW_ERR_ARG, "fused mode %d: expected 0..7", mode);
+    static const int layouts[9] = {0, JW_DEFAULT_LAYOUT, 0, 4, 8, 1, 2, 16, 32};
+    if (mode < 0 || mode > 8) return fail(JW_ERR_ARG, "fused mode %d: expected 0..8", mode);
+    // N = 4096 runs column strips of layouts 0 and 32, N = 8192 of layout 32;
+    // layout 32 exists for those two sizes only
+    const bool large = jw::large_supported(c->N);
+    if (mode > 1 && ((large && !(mode == 8 || (c->N == 4096 && mode == 2))) || (!large && mode == 8)))
+        return fail(JW_ERR_UNSUPPORTED, "fused mode %d is not available at N = %d", mode, c->N);
     c->unfused = mode ? 0 : 1;
     c->layout = layouts[mode];
     return JW_OK;

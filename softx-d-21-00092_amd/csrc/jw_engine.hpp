@@ -543,22 +543,29 @@ template <int N, int BF, typename C>
 struct ColB {
     // BF = 4, 8: one 128-byte segment (SEG elements) holds CPB columns of BF rows;
     // BF = 1 (row-major): 64-byte row segments of CPB adjacent columns;
-    // BF = 16 (row-major): 32-byte row segments (two workgroups per CU in FP64)
-    static constexpr int RB = (BF == 1 || BF == 16) ? 1 : BF;     // rows per block of the layout
-    static constexpr int SEG = BF == 1 ? 64 / (int)sizeof(C) : BF == 16 ? 32 / (int)sizeof(C) : 128 / (int)sizeof(C);
+    // BF = 16 (row-major): 32-byte row segments (two workgroups per CU in FP64
+    // at N = 2048); BF = 32 (row-major): 16-byte row segments, the strip that
+    // fits LDS at N = 8192 (one FP64 or two FP32 columns, 139 KB)
+    static constexpr int RB = (BF == 1 || BF >= 16) ? 1 : BF;     // rows per block of the layout
+    static constexpr int SEG = BF == 1    ? 64 / (int)sizeof(C)
+                               : BF == 16 ? 32 / (int)sizeof(C)
+                               : BF == 32 ? 16 / (int)sizeof(C)
+                                          : 128 / (int)sizeof(C);
     static constexpr int CPB = SEG / RB;               // columns per workgroup
     static constexpr int ELEMS = CPB * N;
-    static constexpr int NTMAX = BF == 1 ? 1024 : 512;
+    static constexpr int NTMAX = (BF == 1 || BF == 32) ? 1024 : 512;
     static constexpr int NT = (ELEMS / 8 < 64) ? 64 : ((ELEMS / 8 > NTMAX) ? NTMAX : ELEMS / 8);
     static constexpr int PER = ELEMS / NT;             // elements per thread
-    // BF = 16 asks for the registers of two workgroups per CU (its LDS allows two)
-    static constexpr int WAVES = BF == 16 ? (NT * 2 + 255) / 256 : 1;
-    static_assert((BF == 1 || BF == 4 || BF == 8 || BF == 16) && CPB >= 1 && CPB <= N && PER * NT == ELEMS &&
-                      N % 16 == 0 && N % CPB == 0,
+    static constexpr int LDS_BYTES = CPB * padded<N, 4>::value * (int)sizeof(C);
+    // the row-major narrow strips ask for the registers of two workgroups per
+    // CU where their LDS allows two
+    static constexpr int WAVES = (BF >= 16 && 2 * LDS_BYTES <= 160 * 1024) ? (NT * 2 + 255) / 256 : 1;
+    static_assert((BF == 1 || BF == 4 || BF == 8 || BF == 16 || BF == 32) && CPB >= 1 && CPB <= N &&
+                      PER * NT == ELEMS && N % 16 == 0 && N % CPB == 0,
                   "blocked column tiling");
 };
 
-template <int N, int BF, int CORR, typename C, int GEN = 0>
+template <int N, int BF, int CORR, typename C, int GEN = 0, int MF = 0>
 __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void k_col_kmult_blk(C* __restrict__ Z, C* __restrict__ X,
                                                                      C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                                      const double* __restrict__ q4,
@@ -578,7 +585,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
     X += zofs(8, V);
     Y += zofs(8, V);
     // 32-byte strips: the four strips of one 128-byte line on one XCD at neighbouring times
-    const int y0 = (BF == 16 ? xcd_strip(blockIdx.x, N / CPB) : (int)blockIdx.x) * CPB;
+    const int y0 = (BF >= 16 ? xcd_strip(blockIdx.x, N / CPB) : (int)blockIdx.x) * CPB;
     const double invV = 1.0 / ((double)N * (double)N);
     // global offset within a plane / LDS slot of loaded element e: x-block
     // e / SEG, slot e mod SEG = c BF + x mod BF (row-major, BF = 1: row e / SEG,
@@ -619,7 +626,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
         if constexpr (!GEN) {
             load(P);
             __syncthreads();
-            lds_fft<N, CPB, 1, PN, NT, -1, 4>(lds, tw_fft(twl), opaque_v(tid));
+            lds_fft<N, CPB, 1, PN, NT, -1, 4, MF>(lds, tw_fft(twl), opaque_v(tid));
         }
         const int t = opaque_v(tid);
 #pragma unroll
@@ -656,7 +663,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
         }
         __syncthreads();
         if (CORR) {
-            lds_fft<N, CPB, 1, PN, NT, +1, 4>(lds, tw_fft(twl), opaque_v(tid));
+            lds_fft<N, CPB, 1, PN, NT, +1, 4, MF>(lds, tw_fft(twl), opaque_v(tid));
             store(P);
             __syncthreads();
         }
@@ -674,7 +681,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
         lds[lpos<1, PN, 4>(pr % N, pr / N)] = acc[q];
     }
     __syncthreads();
-    lds_fft<N, CPB, 1, PN, NT, +1, 4>(lds, tw_fft(twl), opaque_v(tid));
+    lds_fft<N, CPB, 1, PN, NT, +1, 4, MF>(lds, tw_fft(twl), opaque_v(tid));
     store(Z + po);
 }
 
@@ -829,6 +836,18 @@ static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int 
     } else {
         constexpr int CPB = ColB<N, BF, C>::CPB, NT = ColB<N, BF, C>::NT;
         const dim3 grid(N / CPB, 4, nbatch);
+        // the matrix-core stages in the 16-byte strips of the large lattices
+        if constexpr (sizeof(C) == sizeof(float2) && BF == 32) {
+            if (c->fft_mfma) {
+                if (gen)
+                    hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1, C, 1, 1>), grid, dim3(NT), 0, c->stream, JW_COL_ARGS);
+                else if (corr)
+                    hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1, C, 0, 1>), grid, dim3(NT), 0, c->stream, JW_COL_ARGS);
+                else
+                    hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 0, C, 0, 1>), grid, dim3(NT), 0, c->stream, JW_COL_ARGS);
+                return hipGetLastError();
+            }
+        }
         if (gen)
             hipLaunchKernelGGL((k_col_kmult_blk<N, BF, 1, C, 1>), grid, dim3(NT), 0, c->stream, JW_COL_ARGS);
         else if (corr)

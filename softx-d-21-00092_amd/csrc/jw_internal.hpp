@@ -144,6 +144,10 @@ struct StepArgs {
 };
 hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a);
 hipError_t step_fused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a);
+// N = 4096, 8192: fused column passes, separate row transforms and site passes
+bool large_supported(int N);
+hipError_t step_large(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a);
+hipError_t step_large_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a);
 hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32);
 hipError_t step_inc(hipStream_t s, uint32_t* step);
 hipError_t block16(jw_ctx* c, double* out, const double* in);

@@ -7,6 +7,7 @@
 #pragma once
 
 #include "jw_common.hpp"
+#include "jw_engine.hpp"
 
 namespace jw {
 
@@ -286,6 +287,83 @@ static hipError_t step_unfused_t(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, co
     }
 #undef JW_TRY
     return hipSuccess;
+}
+
+
+// ===========================================================================
+// Large lattices (N = 4096, 8192): the fused column passes of jw_engine.hpp
+// (forward column FFT, kernel multiply, inverse column FFT in LDS per strip;
+// the momentum-space noise drawn there) with the row transforms and the site
+// algebra as separate passes, since one lattice row of four planes no longer
+// fits LDS (FP64 at N = 4096 needs 256 KB).  Per step, in complex planes moved:
+//   [position-space noise: noise W 8, forward row FFT R 8 W 8]
+//   column A                     R 8 (0 drawn) W 4 (+8 correlated or drawn)
+//   inverse row FFT of A (and xi~) R 4 (+8) W 4 (+8)
+//   U xi U^dagger                R 9 (+8) W 8
+//   forward row FFT of V         R 8 W 8
+//   column B                     R 8 W 4
+//   inverse row FFT of B         R 4 W 4
+//   update                       R 17 W 9
+// 123 planes with the noise drawn per mode, against 75 for the fused step of
+// N <= 2048.  Column strips: BF = 32 (16-byte row segments) at N = 8192,
+// BF = 0 or 32 at N = 4096 (the intermediate planes stay row-major).
+// ===========================================================================
+template <int N, int BF, typename C>
+static hipError_t step_large_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, const StepArgs& a) {
+    const size_t V = c->V;
+    const C* tw = twiddles_of<C>(c);
+    const unsigned g = grid_for(V, 256);
+    const int wb = a.corr || a.kspace;      // xi~ written back by column A
+    hipError_t e;
+#define JW_TRY(x) if ((e = (x)) != hipSuccess) return e
+    if (!a.kspace) {
+        {
+            KTimer t(c, K_GEN_NOISE);
+            hipLaunchKernelGGL(k_gen_noise_packed<C>, dim3(g), dim3(256), 0, c->stream, Xi, V, a.ns, a.sa_pos);
+            JW_TRY(hipGetLastError());
+        }
+        JW_TRY((launch_rows<N, C>(c, Xi, 8, 1, 1.0, tw)));
+    }
+    {
+        KTimer t(c, K_COL_KMULT);
+        JW_TRY((col_pass<N, BF>(c, ZA, Xi, Xi + 4 * V, a.sa_k, a.sqrt_coupling, a.corr, 1, a.ns, a.kspace, a.corr)));
+    }
+    JW_TRY((launch_rows<N, C>(c, ZA, 4, 0, 1.0, tw)));
+    if (wb) JW_TRY((launch_rows<N, C>(c, Xi, 8, 0, 1.0, tw)));
+    {
+        KTimer t(c, K_UXIU);
+        hipLaunchKernelGGL(k_uxiu_packed<C>, dim3(g), dim3(256), 0, c->stream, Vv, U, wb ? Xi : nullptr, V, a.ns,
+                           a.sa_pos);
+        JW_TRY(hipGetLastError());
+    }
+    JW_TRY((launch_rows<N, C>(c, Vv, 8, 1, 1.0, tw)));
+    {
+        KTimer t(c, K_COL_KMULT);
+        JW_TRY((col_pass<N, BF>(c, ZB, Vv, Vv + 4 * V, a.sa_k, a.sqrt_coupling, 0, 1, a.ns, 0, 0)));
+    }
+    JW_TRY((launch_rows<N, C>(c, ZB, 4, 0, 1.0, tw)));
+    {
+        KTimer t(c, K_UPDATE);
+        hipLaunchKernelGGL(k_update_packed<C>, dim3(g), dim3(256), 0, c->stream, U, ZA, ZB, V, a.sqrt_step, a.reunit);
+        JW_TRY(hipGetLastError());
+    }
+#undef JW_TRY
+    return hipSuccess;
+}
+
+// the column strips of a large lattice: 16-byte segments one plane at a time
+// at N = 8192 (the only strip that fits LDS); at N = 4096 the X + Y tiles of
+// one column (layout 0, default) or one plane at a time over 16-byte segments
+// (layout 32)
+template <typename C>
+static hipError_t step_large_t(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, const StepArgs& a) {
+    if (a.nbatch != 1) return hipErrorInvalidValue;
+    if (c->N == 8192) return step_large_n<8192, 32>(c, U, Xi, Vv, ZA, ZB, a);
+    if (c->N == 4096) {
+        if (c->layout == 32) return step_large_n<4096, 32>(c, U, Xi, Vv, ZA, ZB, a);
+        return step_large_n<4096, 0>(c, U, Xi, Vv, ZA, ZB, a);
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace jw

@@ -10,6 +10,10 @@ hipError_t step_fused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, 
     return step_fused_t(c, U, Xi, Vv, ZA, ZB, a);
 }
 
+hipError_t step_large_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a) {
+    return step_large_t(c, U, Xi, Vv, ZA, ZB, a);
+}
+
 hipError_t step_unfused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const NoiseSrc& ns,
                             const double* sa_pos, const double* sa_k, int kc, int corr, double ss, int reunit) {
     return step_unfused_t(c, U, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc, corr, ss, reunit);

@@ -1,14 +1,21 @@
-// jw_step_f64.hip -- the fused step with FP64 storage (the plain form), and the
+// jw_step_f64.hip -- the fused step with FP64 storage (the plain form), its
+// large-lattice variant (N = 4096, 8192), and the
 // helpers shared by both forms: the 16-row-blocked sqrt(alpha) table and the
 // U conversions of the mixed form.
 #include "jw_engine.hpp"
+#include "jw_packed.hpp"
 
 namespace jw {
 
 bool fused_supported(int N) { return N >= 64 && N <= 2048; }
+bool large_supported(int N) { return N == 4096 || N == 8192; }
 
 hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a) {
     return step_fused_t(c, U, Xi, Vv, ZA, ZB, a);
+}
+
+hipError_t step_large(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a) {
+    return step_large_t(c, U, Xi, Vv, ZA, ZB, a);
 }
 
 __global__ __launch_bounds__(256) void k_block16(double* __restrict__ out, const double* __restrict__ in, int N) {
