@@ -255,8 +255,9 @@ int jw_evolve_noalloc(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double
  * Trajectory t draws the Philox stream of key {seed, traj0 + t}, so it is the
  * trajectory a single lattice evolves after jw_noise_philox(seed, traj0 + t,
  * step).  The intermediate-plane layout of each group is chosen on first use
- * by timing each of layouts 0, 2, 4, 8 and 16 on a copy of its state
- * (jw_batch_set_layout overrides). */
+ * by timing each of layouts 0, 2, 4, 8 and 16, and in FP32 layout 0 with the
+ * matrix-core column stages, on a copy of its state (jw_batch_set_layout
+ * overrides). */
 typedef struct jw_batch jw_batch;
 int jw_batch_create(jw_batch** out, int device, int ngroups, const int* N, const int* count, int fp32);
 void jw_batch_destroy(jw_batch* b);
@@ -266,9 +267,13 @@ int jw_batch_set(jw_batch* b, int t, const jw_field* U);
 int jw_batch_get(jw_batch* b, int t, jw_field* U);
 /* noise key and step of every trajectory (trajectory t: key {seed, traj0 + t}) */
 int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step);
-/* layout 0, 2, 4, 8 or 16 for group g, or -1 to tune on the next evolution; the chosen one */
+/* layout 0, 2, 4, 8 or 16 for group g (FP32 butterflies), or -1 to tune on the
+ * next evolution; the chosen layout, and whether the tuner put the FP32
+ * row-major column transforms' radix-16 stages on the matrix cores (1) or not
+ * (0) (as jw_set_mixed_fft) */
 int jw_batch_set_layout(jw_batch* b, int group, int layout);
 int jw_batch_layout(const jw_batch* b, int group);
+int jw_batch_fft_mode(const jw_batch* b, int group);
 /* JW_NOISE_POSITION or JW_NOISE_MOMENTUM for every trajectory, as
  * jw_set_noise_space (trajectory t keyed {seed, traj0 + t}) */
 int jw_batch_set_noise_space(jw_batch* b, int space);

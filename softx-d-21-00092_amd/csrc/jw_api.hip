@@ -259,6 +259,7 @@ struct BatchGroup {
     void* ZA = nullptr;
     void* ZB = nullptr;
     int layout = -1;           // -1: tune on first use
+    int fft_mfma = 0;          // FP32 groups: radix-16 column stages on the matrix cores (tuned)
     hipEvent_t join = nullptr;
 };
 
@@ -373,8 +374,9 @@ int batch_step_all(jw_batch* b, int coupling, double step, const jw::Alpha& al, 
 }
 
 // The tuner: for each group with no layout yet, time 3 steps (after one
-// untimed) of each intermediate layout on a copy of its U with a scratch noise
-// key, and keep the fastest.  A layout whose tables or steps fail is skipped;
+// untimed) of each intermediate layout (and, in FP32, of the row-major layout
+// with the column transforms' radix-16 stages on the matrix cores) on a copy
+// of its U with a scratch noise key, and keep the fastest.  A layout whose tables or steps fail is skipped;
 // the tuning fails only when no layout ran.  The trajectories and the step
 // counter are left as they were.
 int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
@@ -395,9 +397,14 @@ int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
         const uint32_t saved_traj0 = b->traj0;
         b->traj0 = 0xFFFFF000u;      // scratch noise key
         float best = 1e30f;
-        int best_layout = -1;
-        for (int lay : {0, 2, 4, 8, 16}) {
+        int best_layout = -1, best_mfma = 0;
+        // candidates (layout, matrix-core stages); the stages exist for the FP32 row-major passes
+        static const int cand[6][2] = {{0, 0}, {2, 0}, {4, 0}, {8, 0}, {16, 0}, {0, 1}};
+        for (const auto& cd : cand) {
+            const int lay = cd[0];
+            if (cd[1] && !b->fp32) continue;
             g.ctx->layout = lay;
+            g.ctx->fft_mfma = cd[1];
             const double *sa_pos, *sa_k;
             if (batch_tables(g, coupling, al, &sa_pos, &sa_k) != JW_OK) continue;
             bool ok = batch_group_step(b, g, coupling, step, sa_pos, sa_k, 0, g.count, ucopy, 1, 0) == hipSuccess;
@@ -414,6 +421,7 @@ int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
             if (ms < best) {
                 best = ms;
                 best_layout = lay;
+                best_mfma = cd[1];
             }
         }
         hipEventDestroy(e0);
@@ -423,10 +431,13 @@ int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
         hipFree(ucopy);
         if (best_layout < 0) {
             g.ctx->layout = -1;
+            g.ctx->fft_mfma = 0;
             return fail(JW_ERR_HIP, "tuning: no intermediate layout ran for group %zu", gi);
         }
         g.layout = best_layout;
         g.ctx->layout = best_layout;
+        g.fft_mfma = best_mfma;
+        g.ctx->fft_mfma = best_mfma;
     }
     return JW_OK;
 }
@@ -1285,6 +1296,8 @@ int jw_batch_set_layout(jw_batch* b, int group, int layout) {
     HIP_CHECK(hipStreamSynchronize(b->stream));
     b->g[group].layout = layout;
     b->g[group].ctx->layout = layout;
+    b->g[group].fft_mfma = 0;
+    b->g[group].ctx->fft_mfma = 0;
     if (b->graph) {
         hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -1296,6 +1309,11 @@ int jw_batch_set_layout(jw_batch* b, int group, int layout) {
 int jw_batch_layout(const jw_batch* b, int group) {
     if (!b || group < 0 || group >= (int)b->g.size()) return -2;
     return b->g[group].layout;
+}
+
+int jw_batch_fft_mode(const jw_batch* b, int group) {
+    if (!b || group < 0 || group >= (int)b->g.size()) return -2;
+    return b->g[group].fft_mfma;
 }
 
 int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw_alpha* a, int nsteps,

@@ -476,12 +476,14 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, 
             vx = lds[lpos<S2, 1, 3>(kx, c)];
             vy = lds[lpos<S2, 1, 3>(kx, W + c)];
         }
-        double fx = 0.0, fy = 0.0;
+        // kernel factors in the element type: one FP64 division per pair in FP64,
+        // one FP32 division in the mixed form (whose spectra are FP32)
+        T fx = 0, fy = 0;
         if (kx != 0 || ky != 0) {
-            const double ph = q4[kx] + q4[ky];
             const double cc = sqrt_coupling ? sa[k] : 1.0;
-            fx = -2.0 * M_PI * cc * sb[kx] / ph * invV;
-            fy = -2.0 * M_PI * cc * sb[ky] / ph * invV;
+            const T f = (T)(-2.0 * M_PI * invV * cc) / (T)(q4[kx] + q4[ky]);
+            fx = f * (T)sb[kx];
+            fy = f * (T)sb[ky];
         }
         if (CORR) {
             if (noise_scale) {
@@ -652,11 +654,10 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
                 if (noise_scale) v = cscale(v, (T)sad[k16]);
                 lds[l] = cscale(v, (T)invV);
             }
-            double f = 0.0;
+            T f = 0;
             if (kx != 0 || ky != 0) {
-                const double ph = q4d[kx] + q4d[ky];
                 const double cc = sqrt_coupling ? sad[k16] : 1.0;
-                f = -2.0 * M_PI * cc * sbd[d ? ky : kx] / ph * invV;
+                f = (T)(-2.0 * M_PI * invV * cc) / (T)(q4d[kx] + q4d[ky]) * (T)sbd[d ? ky : kx];
             }
             if (d == 0) acc[q] = mkc<C>(-(f * v.y), f * v.x);
             else acc[q] = mkc<C>(acc[q].x - f * v.y, acc[q].y + f * v.x);

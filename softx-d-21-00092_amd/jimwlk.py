@@ -108,6 +108,7 @@ def lib():
         L.jw_batch_philox.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.jw_batch_set_layout.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.jw_batch_layout.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_batch_fft_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_batch_evolve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                       ctypes.POINTER(Alpha), ctypes.c_int, ctypes.c_int]
         L.jw_batch_sync.argtypes = [ctypes.c_void_p]
@@ -386,6 +387,10 @@ class Batch:
 
     def layout(self, group):
         return lib().jw_batch_layout(self.h, group)
+
+    def fft_mode(self, group):
+        """1 when the tuner put the group's FP32 radix-16 column stages on the matrix cores."""
+        return lib().jw_batch_fft_mode(self.h, group)
 
     def evolve(self, coupling=FIXED_COUPLING, kernel=SIN_KERNEL, step=1e-4, al=None, nsteps=1, reunit_every=10):
         _check(lib().jw_batch_evolve(self.h, coupling, kernel, step, ctypes.byref(al) if al else None, nsteps,
