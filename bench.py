@@ -249,6 +249,7 @@ def main():
         b.sync()
         dt = time.perf_counter() - t0
         lays = [b.layout(i) for i in range(len(groups))]
+        lays = [(l, "matrix-core" if b.fft_mode(i) == 1 else "butterflies") if fp32 else l for i, l in enumerate(lays)]
         b.close()
         for c_ in ctxs.values():
             c_.close()

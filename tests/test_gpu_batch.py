@@ -151,11 +151,15 @@ def test_batch_tuned_layout(jw):
     b.sync()
     for g in range(len(groups)):
         assert b.layout(g) in (0, 2, 4, 8, 16)
+        assert b.fft_mode(g) in (0, 1) and (b.fft_mode(g) == 0 or b.layout(g) == 0)
+    gi = [g for g, (N, count) in enumerate(groups) for _ in range(count)]
     for t, (N, U0) in enumerate(states):
         c = ctxs[N]
         got = c.field()
         b.get(t, got)
+        c.set_mixed_fft(b.fft_mode(gi[t]))          # the tuner may pick the matrix-core stages
         ref = single(jw, c, U0, 1, 11, t, 0, 12)
+        c.set_mixed_fft(0)
         assert frob_max(got.download(), ref) <= 2e-6, t
     b.close()
     for c in ctxs.values():
