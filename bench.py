@@ -90,7 +90,7 @@ def parse():
     p.add_argument("--calibrate", action="store_true",
                    help="also run a plain copy of 8 planes (k_calibration_copy) to scale traffic counters")
     p.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
-    p.add_argument("--cpu-steps-shipped", type=int, default=1,
+    p.add_argument("--cpu-steps-shipped", type=int, default=2,
                    help="CPU baseline steps with the reference's threading (serial uxiulocal/update_uf)")
     p.add_argument("--cpu-steps-omp", type=int, default=2, help="CPU baseline steps with every loop threaded")
     p.add_argument("--fused-mode", type=int, default=1,
@@ -222,9 +222,12 @@ def main():
                                        "reals_per_site": REALS_PER_SITE[kind_f], "bytes_per_real": bpr}}
         F.close()
 
-    def time_batch(groups, fp32, coupling_b, step_b, al_b, nsteps, warm, seeds0, kspace=False):
-        """Aggregate rate of a batch of MV starts (own seed each): (ensemble steps/s, layouts)."""
+    def time_batch(groups, fp32, coupling_b, step_b, al_b, nsteps, warm, seeds0, kspace=False, launch=-1):
+        """Aggregate rate of a batch of MV starts (own seed each): (ensemble steps/s, layouts,
+        launch mode: 0 per group, 1 one launch per kernel class)."""
         b = jw.Batch(groups, fp32=fp32)
+        if launch >= 0:
+            b.set_launch(launch)
         if kspace:
             b.set_noise_space(jw.NOISE_MOMENTUM)
         ctxs = {}
@@ -250,16 +253,18 @@ def main():
         dt = time.perf_counter() - t0
         lays = [b.layout(i) for i in range(len(groups))]
         lays = [(l, "matrix-core" if b.fft_mode(i) == 1 else "butterflies") if fp32 else l for i, l in enumerate(lays)]
+        mode = b.launch()
         b.close()
         for c_ in ctxs.values():
             c_.close()
-        return nsteps / dt, lays
+        return nsteps / dt, lays, mode
 
     batched = None
     if a.batch > 0:
-        sps_b, lays = time_batch([(N, a.batch)], True, coupling, step, al, a.steps, a.warmup, 1000, ks)
+        sps_b, lays, mode = time_batch([(N, a.batch)], True, coupling, step, al, a.steps, a.warmup, 1000, ks)
         batched = {"trajectories": a.batch, "N": N, "ensemble_steps_per_s": round(sps_b, 3),
-                   "traj_steps_per_s": round(sps_b * a.batch, 3), "layout": lays[0], "noise_space": a.noise_space}
+                   "traj_steps_per_s": round(sps_b * a.batch, 3), "layout": lays[0], "launch": mode,
+                   "noise_space": a.noise_space}
         print(f"bench: mixed+batched: {a.batch} x N={N}: {sps_b * a.batch:.1f} trajectory-steps/s "
               f"(layout {lays[0]})", file=sys.stderr, flush=True)
     ensemble = None
@@ -269,11 +274,11 @@ def main():
         ensemble = {"groups": [{"N": n, "count": c} for n, c in groups], "sites": sites,
                     "coupling": "fixed", "step": 1e-4, "noise_space": a.noise_space}
         for fp32, name in ((True, "mixed"), (False, "fp64")):
-            sps_e, lays = time_batch(groups, fp32, jw.FIXED_COUPLING, 1e-4, None, 20, 10, 2000, ks)
+            sps_e, lays, mode = time_batch(groups, fp32, jw.FIXED_COUPLING, 1e-4, None, 20, 10, 2000, ks)
             ensemble[name] = {"steps_per_s": round(sps_e, 3), "site_updates_per_s": round(sps_e * sites, 1),
-                              "layouts": lays}
+                              "layouts": lays, "launch": mode}
             print(f"bench: ensemble {name}: {sps_e:.1f} steps/s = {sps_e * sites:.3e} site-updates/s "
-                  f"(layouts {lays})", file=sys.stderr, flush=True)
+                  f"(layouts {lays}, launch mode {mode})", file=sys.stderr, flush=True)
         ensemble["mixed_over_fp64"] = round(ensemble["mixed"]["steps_per_s"] / ensemble["fp64"]["steps_per_s"], 3)
 
     # the large single lattice: N = 8192, MV start Ny = 100, g^2 mu a = 0.05,

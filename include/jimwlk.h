@@ -250,8 +250,9 @@ int jw_evolve_noalloc(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double
  * (64 <= N <= 2048): group i holds count[i] trajectories of N[i]^2, numbered
  * consecutively over the groups.  fp32 = 1: the mixed-precision form (as
  * jw_evolve_mixed), 0: FP64 (as jw_evolve).  Each group's kernels take all its
- * trajectories in one launch; one step of all groups (their launches forked
- * onto one stream per group) is captured once into a HIP graph and replayed.
+ * trajectories in one launch, or every group's in one launch per kernel class
+ * (jw_batch_set_launch); one step of all groups is captured once into a HIP
+ * graph and replayed.
  * Trajectory t draws the Philox stream of key {seed, traj0 + t}, so it is the
  * trajectory a single lattice evolves after jw_noise_philox(seed, traj0 + t,
  * step).  The intermediate-plane layout of each group is chosen on first use
@@ -274,6 +275,15 @@ int jw_batch_philox(jw_batch* b, uint32_t seed, uint32_t traj0, uint32_t step);
 int jw_batch_set_layout(jw_batch* b, int group, int layout);
 int jw_batch_layout(const jw_batch* b, int group);
 int jw_batch_fft_mode(const jw_batch* b, int group);
+/* How a step launches its kernels: 0, each size group its own launches on its
+ * own stream (the groups run concurrently; per-group layouts); 1, one launch
+ * per kernel class over all groups (at most JW_MAX_GROUPS = 8; row-major
+ * planes, FP32 butterflies): block b of a launch finds its group from the
+ * groups' block ranges, and a group of small N runs several rows or column
+ * strips side by side in one workgroup; -1 (default): timed on the first
+ * evolution after the layouts, the faster kept.  jw_batch_launch: the mode. */
+int jw_batch_set_launch(jw_batch* b, int mode);
+int jw_batch_launch(const jw_batch* b);
 /* JW_NOISE_POSITION or JW_NOISE_MOMENTUM for every trajectory, as
  * jw_set_noise_space (trajectory t keyed {seed, traj0 + t}) */
 int jw_batch_set_noise_space(jw_batch* b, int space);

@@ -279,6 +279,7 @@ struct BatchKey {
 struct jw_batch {
     int device = 0, fp32 = 0, ntraj = 0;
     int kspace = 0;                  // momentum-space noise (jw_batch_set_noise_space)
+    int launch = -1;                 // 0: per group, 1: one launch per kernel class, -1: tune
     std::vector<BatchGroup> g;
     hipStream_t stream = nullptr;    // origin of the step graph; groups fork from it
     hipEvent_t fork = nullptr;
@@ -359,17 +360,112 @@ hipError_t batch_group_step(jw_batch* b, BatchGroup& g, int coupling, double ste
 // Each group stream waits on the fork before anything is enqueued on it, the
 // coupling-table check included (the tables are built before a capture, so
 // inside one nothing is rebuilt).
-int batch_step_all(jw_batch* b, int coupling, double step, const jw::Alpha& al, int reunit, int first, int next) {
-    HIP_CHECK(hipEventRecord(b->fork, b->stream));
-    for (auto& g : b->g) {
-        const double *sa_pos, *sa_k;
-        HIP_CHECK(hipStreamWaitEvent(g.ctx->stream, b->fork, 0));
-        if (int e = batch_tables(g, coupling, al, &sa_pos, &sa_k)) return e;
-        HIP_CHECK(batch_group_step(b, g, coupling, step, sa_pos, sa_k, reunit, g.count, g.U, first, next));
-        HIP_CHECK(hipEventRecord(g.join, g.ctx->stream));
-        HIP_CHECK(hipStreamWaitEvent(b->stream, g.join, 0));
+//
+// launch 1: all groups in one launch per kernel class on b->stream (their
+// planes row-major); Uv: the groups' U arrays (nullptr: their own); inc: the
+// step counter advances (not while tuning).
+int batch_step_all(jw_batch* b, int coupling, double step, const jw::Alpha& al, int reunit, int first, int next,
+                   void* const* Uv = nullptr, bool inc = true, int launch = -1) {
+    if (launch < 0) launch = b->launch;
+    if (launch == 1) {
+        const int ng = (int)b->g.size();
+        if (ng > JW_MAX_GROUPS) return fail(JW_ERR_UNSUPPORTED, "one launch per kernel class holds %d groups", JW_MAX_GROUPS);
+        jw::GroupArgs ga[JW_MAX_GROUPS];
+        for (int gi = 0; gi < ng; gi++) {
+            BatchGroup& g = b->g[gi];
+            const double *sa_pos, *sa_k;
+            if (int e = batch_tables(g, coupling, al, &sa_pos, &sa_k)) return e;
+            const jw::NoiseSrc ns{nullptr, b->seed, b->traj0 + (uint32_t)g.first, 0u,
+                                  b->kspace ? JW_TAG_KSPACE : JW_TAG_LANGEVIN, b->d_step};
+            ga[gi] = jw::GroupArgs{g.N,    g.count, Uv ? Uv[gi] : g.U, g.Xi,         g.Vv,     g.ZA,     g.ZB,
+                                   g.ctx->V, ns,    sa_pos,           sa_k,         g.ctx->q4, g.ctx->sb,
+                                   b->fp32 ? (const void*)g.ctx->twf : (const void*)g.ctx->tw};
+        }
+        const jw::StepArgs a{ga[0].ns, nullptr, nullptr, coupling == JW_SQRT_COUPLING_CONSTANT,
+                             coupling == JW_CORRELATED_NOISE_COUPLING, sqrt(step), reunit, 1, 1, 0, b->kspace};
+        HIP_CHECK(b->fp32 ? jw::step_multi_f32(b->stream, ga, ng, a) : jw::step_multi(b->stream, ga, ng, a));
+    } else {
+        HIP_CHECK(hipEventRecord(b->fork, b->stream));
+        for (size_t gi = 0; gi < b->g.size(); gi++) {
+            BatchGroup& g = b->g[gi];
+            const double *sa_pos, *sa_k;
+            HIP_CHECK(hipStreamWaitEvent(g.ctx->stream, b->fork, 0));
+            if (int e = batch_tables(g, coupling, al, &sa_pos, &sa_k)) return e;
+            HIP_CHECK(batch_group_step(b, g, coupling, step, sa_pos, sa_k, reunit, g.count, Uv ? Uv[gi] : g.U, first,
+                                       next));
+            HIP_CHECK(hipEventRecord(g.join, g.ctx->stream));
+            HIP_CHECK(hipStreamWaitEvent(b->stream, g.join, 0));
+        }
     }
-    HIP_CHECK(jw::step_inc(b->stream, b->d_step));
+    if (inc) HIP_CHECK(jw::step_inc(b->stream, b->d_step));
+    return JW_OK;
+}
+
+// the row-major planes and FP32 butterflies that one launch per kernel class runs
+void batch_rowmajor(jw_batch* b) {
+    for (auto& g : b->g) {
+        g.layout = 0;
+        g.ctx->layout = 0;
+        g.fft_mfma = 0;
+        g.ctx->fft_mfma = 0;
+    }
+}
+
+// The launch mode: time 3 steps (after one untimed) of the whole batch with the
+// tuned per-group launches and with one launch per kernel class (row-major
+// planes), on copies of every trajectory with a scratch noise key; keep the
+// faster.  The trajectories and the step counter are left as they were.
+int batch_tune_launch(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
+    const int ng = (int)b->g.size();
+    if (ng > JW_MAX_GROUPS) {
+        b->launch = 0;
+        return JW_OK;
+    }
+    std::vector<void*> uc(ng, nullptr);
+    std::vector<int> lay(ng), mf(ng);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = JW_OK;
+    for (auto& g : b->g) HIP_CHECK(hipStreamSynchronize(g.ctx->stream));
+    for (int gi = 0; gi < ng && rc == JW_OK; gi++) {
+        const size_t bytes = (size_t)b->g[gi].count * 9 * b->g[gi].ctx->V * elem_size(b);
+        if (hipMalloc(&uc[gi], bytes) != hipSuccess ||
+            hipMemcpyAsync(uc[gi], b->g[gi].U, bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+            rc = fail(JW_ERR_ALLOC, "launch tuning buffer allocation failed");
+        lay[gi] = b->g[gi].layout;
+        mf[gi] = b->g[gi].fft_mfma;
+    }
+    if (rc == JW_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(JW_ERR_HIP, "launch tuning events");
+    const uint32_t saved_traj0 = b->traj0;
+    b->traj0 = 0xFFFFF000u;      // scratch noise key
+    float ms[2] = {1e30f, 1e30f};
+    for (int mode = 0; mode < 2 && rc == JW_OK; mode++) {
+        if (mode == 1) batch_rowmajor(b);
+        bool ok = batch_step_all(b, coupling, step, al, 0, 1, 0, uc.data(), false, mode) == JW_OK;
+        ok = ok && hipEventRecord(e0, b->stream) == hipSuccess;
+        for (int it = 0; it < 3 && ok; it++) ok = batch_step_all(b, coupling, step, al, 0, 1, 0, uc.data(), false, mode) == JW_OK;
+        ok = ok && hipEventRecord(e1, b->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms[mode], e0, e1) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            ms[mode] = 1e30f;
+        }
+    }
+    b->traj0 = saved_traj0;
+    hipStreamSynchronize(b->stream);
+    for (auto& g : b->g) hipStreamSynchronize(g.ctx->stream);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    for (void* p : uc)
+        if (p) hipFree(p);
+    if (rc != JW_OK) return rc;
+    if (ms[0] >= 1e30f && ms[1] >= 1e30f) return fail(JW_ERR_HIP, "launch tuning: neither launch mode ran");
+    b->launch = ms[1] < ms[0] ? 1 : 0;
+    if (b->launch == 0)      // back to the tuned per-group layouts
+        for (int gi = 0; gi < ng; gi++) {
+            b->g[gi].layout = b->g[gi].ctx->layout = lay[gi];
+            b->g[gi].fft_mfma = b->g[gi].ctx->fft_mfma = mf[gi];
+        }
     return JW_OK;
 }
 
@@ -1294,6 +1390,7 @@ int jw_batch_set_layout(jw_batch* b, int group, int layout) {
     if (layout != -1 && layout != 0 && layout != 2 && layout != 4 && layout != 8 && layout != 16)
         return fail(JW_ERR_ARG, "layout %d", layout);
     HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (layout > 0) b->launch = 0;      // one launch per kernel class runs row-major planes
     b->g[group].layout = layout;
     b->g[group].ctx->layout = layout;
     b->g[group].fft_mfma = 0;
@@ -1310,6 +1407,24 @@ int jw_batch_layout(const jw_batch* b, int group) {
     if (!b || group < 0 || group >= (int)b->g.size()) return -2;
     return b->g[group].layout;
 }
+
+int jw_batch_set_launch(jw_batch* b, int mode) {
+    if (!b) return fail(JW_ERR_ARG, "null batch");
+    if (mode < -1 || mode > 1) return fail(JW_ERR_ARG, "launch mode %d: expected -1, 0 or 1", mode);
+    if (mode == 1 && (int)b->g.size() > JW_MAX_GROUPS)
+        return fail(JW_ERR_UNSUPPORTED, "one launch per kernel class holds %d groups", JW_MAX_GROUPS);
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    b->launch = mode;
+    if (mode == 1) batch_rowmajor(b);
+    if (b->graph) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->have_key = false;
+    }
+    return JW_OK;
+}
+
+int jw_batch_launch(const jw_batch* b) { return b ? b->launch : -2; }
 
 int jw_batch_fft_mode(const jw_batch* b, int group) {
     if (!b || group < 0 || group >= (int)b->g.size()) return -2;
@@ -1339,6 +1454,8 @@ int jw_batch_evolve(jw_batch* b, int coupling, int kernel, double step, const jw
     // that graph: wait for the previous call's work first.
     if (!same_key) HIP_CHECK(hipStreamSynchronize(b->stream));
     if (int e = batch_tune(b, coupling, step, al)) return e;
+    if (b->launch < 0)
+        if (int e = batch_tune_launch(b, coupling, step, al)) return e;
     if (b->graph && !same_key) {
         hipGraphExecDestroy(b->graph);
         b->graph = nullptr;

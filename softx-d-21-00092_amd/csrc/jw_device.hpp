@@ -707,8 +707,13 @@ __device__ __forceinline__ auto tw_source(const C* __restrict__ tw);
 // kept live from one FFT to the next), LDS tables as they are
 template <typename C>
 __device__ __forceinline__ const C* tw_fft(const C* p) {
-    asm volatile("" : "+s"(p));
-    return p;
+    // through a global-memory pointer, so that loads after the empty asm stay
+    // global loads whatever the compiler knew of p (flat loads would also count
+    // against the LDS wait counter)
+    typedef const __attribute__((address_space(1))) C GC;
+    GC* g = (GC*)p;
+    asm volatile("" : "+s"(g));
+    return (const C*)g;
 }
 template <typename C, int N>
 __device__ __forceinline__ TwL<C, N> tw_fft(TwL<C, N> t) { return t; }

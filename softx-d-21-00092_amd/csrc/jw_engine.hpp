@@ -111,6 +111,23 @@ __device__ __forceinline__ NoiseSrc batch_noise(NoiseSrc ns, size_t V) {
     return ns;
 }
 __device__ __forceinline__ size_t zofs(int planes, size_t V) { return (size_t)blockIdx.z * planes * V; }
+// the same for an explicit trajectory z (kernel bodies run by one workgroup or,
+// in a launch over several size groups, by a part of one)
+__device__ __forceinline__ NoiseSrc batch_noise_z(NoiseSrc ns, size_t V, int z) {
+    ns.traj += z;
+    if (ns.inj) ns.inj += (size_t)z * 16 * V;
+    if (ns.stepp) ns.step = *ns.stepp;
+    return ns;
+}
+__device__ __forceinline__ size_t zofs_z(int z, int planes, size_t V) { return (size_t)z * planes * V; }
+// where a kernel body runs: its row, strip or segment (bx), plane pair (by),
+// trajectory (bz) and thread index among the threads that run it (tid)
+struct BlockIx {
+    int bx, by, bz, tid;
+};
+__device__ __forceinline__ BlockIx own_block() {
+    return BlockIx{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)threadIdx.x};
+}
 
 // matrix type of the site algebra's products: FP64 in the FP64 form, FP32 in
 // the mixed form (whose exponentials and projection stay FP64)
@@ -131,8 +148,10 @@ __device__ __forceinline__ void fence(double2& v) { asm volatile("" : "+v"(v.x),
 // one FFT live in registers until the next)
 template <typename T>
 __device__ __forceinline__ const T* opaque(const T* p) {
-    asm volatile("" : "+s"(p));
-    return p;
+    typedef const __attribute__((address_space(1))) T GT;     // a global-memory table
+    GT* g = (GT*)p;
+    asm volatile("" : "+s"(g));
+    return (const T*)g;
 }
 // likewise for a per-thread index: values derived from it are recomputed
 // rather than kept live from one phase of a kernel to the next
@@ -166,11 +185,10 @@ __device__ __forceinline__ int site_of(int y) {
 // ---------------------------------------------------------------------------
 template <int N, int BF, typename C, typename TW>
 __device__ __forceinline__ void row_noise_body(C* lds, C* __restrict__ Xi, size_t V, const NoiseSrc& ns,
-                                               const double* __restrict__ sa_pos, TW tw, int x) {
+                                               const double* __restrict__ sa_pos, TW tw, int x, int tid) {
     typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
-    const int tid = threadIdx.x;
     const size_t row0 = (size_t)x * N;
     T m[SPT][8];
 #pragma unroll
@@ -215,19 +233,18 @@ template <int N, typename C>
 struct NoiseWaves { static constexpr int value = sizeof(C) == sizeof(double2) ? 4 : 8; };
 
 template <int N, int BF, typename C>
-__global__ __launch_bounds__(RowE<N>::NT, (NoiseWaves<N, C>::value)) void k_row_noise_fwd(C* __restrict__ Xi, size_t V, NoiseSrc ns,
+__device__ __forceinline__ void row_noise_fwd_body(C* lds, BlockIx bi, C* __restrict__ Xi, size_t V, NoiseSrc ns,
                                                               const double* __restrict__ sa_pos,
                                                               const C* __restrict__ tw) {
     typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;
-    __shared__ C lds[2 * PN];
     const auto twl = tw_source<N, true>(tw);
-    Xi += zofs(8, V);
-    ns = batch_noise(ns, V);
-    const int x = row_of_block<N, BF>(blockIdx.x);
+    Xi += zofs_z(bi.bz, 8, V);
+    ns = batch_noise_z(ns, V, bi.bz);
+    const int x = row_of_block<N, BF>(bi.bx);
     const size_t row0 = (size_t)x * N;
-    const int tid = threadIdx.x;
+    const int tid = bi.tid;
 #pragma unroll 1
     for (int p = 0; p < 4; p++) {
 #pragma unroll
@@ -262,24 +279,31 @@ __global__ __launch_bounds__(RowE<N>::NT, (NoiseWaves<N, C>::value)) void k_row_
     }
 }
 
+template <int N, int BF, typename C>
+__global__ __launch_bounds__(RowE<N>::NT, (NoiseWaves<N, C>::value)) void k_row_noise_fwd(C* __restrict__ Xi, size_t V, NoiseSrc ns,
+                                                              const double* __restrict__ sa_pos,
+                                                              const C* __restrict__ tw) {
+    __shared__ C lds[2 * padded<N, 4>::value];
+    row_noise_fwd_body<N, BF, C>(lds, own_block(), Xi, V, ns, sa_pos, tw);
+}
+
 // ---------------------------------------------------------------------------
 // row pass 2: V_i = U xi_i U^dagger -> V row spectra (planes 0..3 V_x, 4..7 V_y),
 // xi regenerated from the noise source (white noise)
 // ---------------------------------------------------------------------------
 template <int N, int BF, typename C>
-__global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_fwd(C* __restrict__ Vv, const C* __restrict__ U,
+__device__ __forceinline__ void row_uxiu_fwd_body(C* lds, BlockIx bi, C* __restrict__ Vv, const C* __restrict__ U,
                                                              size_t V, NoiseSrc ns, const double* __restrict__ sa_pos,
                                                              const C* __restrict__ tw) {
     typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
-    __shared__ C lds[4 * PN];
-    const int tid = threadIdx.x;
+    const int tid = bi.tid;
     const auto twl = tw_source<N, true>(tw);
-    Vv += zofs(8, V);
-    U += zofs(9, V);
-    ns = batch_noise(ns, V);
-    const int x = row_of_block<N, BF>(blockIdx.x);
+    Vv += zofs_z(bi.bz, 8, V);
+    U += zofs_z(bi.bz, 9, V);
+    ns = batch_noise_z(ns, V, bi.bz);
+    const int x = row_of_block<N, BF>(bi.bx);
     const size_t row0 = (size_t)x * N;
     T vy[SPT][8];      // in the element type: they go to LDS in it
 #pragma unroll
@@ -327,6 +351,14 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_fw
             Vv[pat<N, BF>(4 + p, x, tid + q * NT, V)] = lds[lpos<1, PN, 4>(tid + q * NT, p)];
 }
 
+template <int N, int BF, typename C>
+__global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_fwd(C* __restrict__ Vv, const C* __restrict__ U,
+                                                             size_t V, NoiseSrc ns, const double* __restrict__ sa_pos,
+                                                             const C* __restrict__ tw) {
+    __shared__ C lds[4 * padded<N, 4>::value];
+    row_uxiu_fwd_body<N, BF, C>(lds, own_block(), Vv, U, V, ns, sa_pos, tw);
+}
+
 // ---------------------------------------------------------------------------
 // row pass 2, correlated noise: reads the column-inverse spectra of xi~,
 // finishes their inverse transform along the row, and forms V_i = U xi~_i U^dagger
@@ -336,19 +368,18 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_fw
 // and measured slower (one workgroup per CU instead of two in FP32).
 // ---------------------------------------------------------------------------
 template <int N, int BF, typename C>
-__global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_corr(C* __restrict__ Vv, const C* __restrict__ U,
+__device__ __forceinline__ void row_uxiu_corr_body(C* lds, BlockIx bi, C* __restrict__ Vv, const C* __restrict__ U,
                                                               const C* __restrict__ Xs, size_t V,
                                                               const C* __restrict__ tw) {
     typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N>::NT, SPT = RowE<N>::SPT;
     constexpr int PN = padded<N, 4>::value;
-    __shared__ C lds[4 * PN];
-    const int tid = threadIdx.x;
+    const int tid = bi.tid;
     const auto twl = tw_source<N, false>(tw);
-    Vv += zofs(8, V);
-    U += zofs(9, V);
-    Xs += zofs(8, V);
-    const int x = row_of_block<N, BF>(blockIdx.x);
+    Vv += zofs_z(bi.bz, 8, V);
+    U += zofs_z(bi.bz, 9, V);
+    Xs += zofs_z(bi.bz, 8, V);
+    const int x = row_of_block<N, BF>(bi.bx);
     const size_t row0 = (size_t)x * N;
 #pragma unroll 1
     for (int half = 0; half < 2; half++) {
@@ -390,6 +421,14 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
     }
 }
 
+template <int N, int BF, typename C>
+__global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_corr(C* __restrict__ Vv, const C* __restrict__ U,
+                                                              const C* __restrict__ Xs, size_t V,
+                                                              const C* __restrict__ tw) {
+    __shared__ C lds[4 * padded<N, 4>::value];
+    row_uxiu_corr_body<N, BF, C>(lds, own_block(), Vv, U, Xs, V, tw);
+}
+
 // ---------------------------------------------------------------------------
 // column pass: Z[p] = F_col^-1[ i (kx X^[p] + ky Y^[p]) / V ], X^ = F_col X
 //   kx = -2 pi c pbarX / phat2 (c = sa(k) for SQRT, else 1), zero at k = 0.
@@ -408,7 +447,7 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
 // workgroups asked for, as the FP32 butterfly version fits them by itself; the
 // correlated variant needs more than that without spilling and runs one per CU
 template <int N, int CORR, typename C, int PI = 0, int MF = 0, int GEN = 0>
-__global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, C, CORR>::NT / 256 : 1)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
+__device__ __forceinline__ void col_kmult_body(C* lds, BlockIx bi, C* __restrict__ Z, C* __restrict__ X,
                                                           C* __restrict__ Y, size_t V, int sqrt_coupling,
                                                           const double* __restrict__ q4,
                                                           const double* __restrict__ sb,
@@ -419,15 +458,14 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, 
     typedef ColE<N, C, CORR> CE;
     constexpr int W = CE::W, NT = CE::NT, PER = CE::PER, PAIRS = CE::PAIRS;
     constexpr int S2 = 2 * W;
-    __shared__ C lds[padded<N, 3>::value * S2];
-    const int tid = threadIdx.x;
+    const int tid = bi.tid;
     const auto twl = tw_source<N, sizeof(C) == sizeof(float2)>(tw);
-    const int strip = xcd_strip(blockIdx.x, N / W);
+    const int strip = xcd_strip(bi.bx, N / W);
     const int col0 = strip * W;
-    Z += zofs(4, V);
-    X += zofs(8, V);
-    Y += zofs(8, V);
-    const size_t pofs = (size_t)blockIdx.y * V;
+    Z += zofs_z(bi.bz, 4, V);
+    X += zofs_z(bi.bz, 8, V);
+    Y += zofs_z(bi.bz, 8, V);
+    const size_t pofs = (size_t)bi.by * V;
     const double invV = 1.0 / ((double)N * (double)N);
     // element e of the tile: row e / 2W, LDS slot s (X for s < W, Y above),
     // global offset g within X (PI: the pair-interleaved array) or Y
@@ -453,7 +491,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, 
         __syncthreads();
         lds_fft<N, S2, S2, 1, NT, -1, 3, MF>(lds, tw_fft(twl), opaque_v(tid));
     }
-    const NoiseSrc nk = batch_noise(ns, V);
+    const NoiseSrc nk = batch_noise_z(ns, V, bi.bz);
     // pair (x, c): X^ in slot c of LDS row x, Y^ in slot W + c
     C zreg[CORR ? PAIRS : 1];
     // FP32 correlated: indices recomputed per phase (opaque_v), since addresses
@@ -469,7 +507,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, 
         C vx, vy;
         if constexpr (GEN) {
             T g[4];
-            kspace_normals4((uint32_t)k, nk.step, nk.tag, blockIdx.y, nk.seed, nk.traj, g);
+            kspace_normals4((uint32_t)k, nk.step, nk.tag, bi.by, nk.seed, nk.traj, g);
             vx = mkc<C>(g[0] * (T)N, g[1] * (T)N);          // sqrt(V) = N
             vy = mkc<C>(g[2] * (T)N, g[3] * (T)N);
         } else {
@@ -529,6 +567,17 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, 
         const int x = pr / W, c = pr % W;
         Z[pofs + (size_t)x * N + col0 + c] = lds[lpos<S2, 1, 3>(x, c)];
     }
+}
+
+template <int N, int CORR, typename C, int PI = 0, int MF = 0, int GEN = 0>
+__global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, C, CORR>::NT / 256 : 1)) void k_col_kmult(C* __restrict__ Z, C* __restrict__ X,
+                                                          C* __restrict__ Y, size_t V, int sqrt_coupling,
+                                                          const double* __restrict__ q4,
+                                                          const double* __restrict__ sb,
+                                                          const double* __restrict__ sa,
+                                                          const C* __restrict__ tw, NoiseSrc ns, int noise_scale) {
+    __shared__ C lds[padded<N, 3>::value * 2 * ColE<N, C, CORR>::W];
+    col_kmult_body<N, CORR, C, PI, MF, GEN>(lds, own_block(), Z, X, Y, V, sqrt_coupling, q4, sb, sa, tw, ns, noise_scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -695,7 +744,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
 // forms; U is stored in the element type C.
 // ---------------------------------------------------------------------------
 template <int N, int BF, typename C>
-__global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_update(C* __restrict__ U, const C* __restrict__ ZA,
+__device__ __forceinline__ void row_update_body(C* lds, BlockIx bi, C* __restrict__ U, const C* __restrict__ ZA,
                                                               const C* __restrict__ ZB, size_t V, double ss,
                                                               int reunit, const C* __restrict__ tw, int next_noise,
                                                               C* __restrict__ Xi, NoiseSrc ns,
@@ -704,13 +753,12 @@ __global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_up
     typedef typename scalar_of<C>::type T;
     constexpr int NT = RowE<N, 4>::NT, SPT = RowE<N, 4>::SPT;
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
-    __shared__ C lds[4 * PN];
-    const int tid = threadIdx.x;
+    const int tid = bi.tid;
     const auto twl = tw_source<N, sizeof(C) == sizeof(float2)>(tw);
-    U += zofs(9, V);
-    ZA += zofs(4, V);
-    ZB += zofs(4, V);
-    const int x = row_of_block<N, BF>(blockIdx.x);
+    U += zofs_z(bi.bz, 9, V);
+    ZA += zofs_z(bi.bz, 4, V);
+    ZB += zofs_z(bi.bz, 4, V);
+    const int x = row_of_block<N, BF>(bi.bx);
     const size_t row0 = (size_t)x * N;
 #pragma unroll
     for (int p = 0; p < 4; p++)
@@ -786,10 +834,246 @@ __global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_up
     }
     if (next_noise) {        // the next step's xi row spectra
         __syncthreads();
-        NoiseSrc nn = batch_noise(ns, V);
+        NoiseSrc nn = batch_noise_z(ns, V, bi.bz);
         nn.step += 1;
-        row_noise_body<N, BF, C>(lds, Xi + zofs(8, V), V, nn, sa_pos, tw_fft(twl), x);
+        row_noise_body<N, BF, C>(lds, Xi + zofs_z(bi.bz, 8, V), V, nn, sa_pos, tw_fft(twl), x, bi.tid);
     }
+}
+
+template <int N, int BF, typename C>
+__global__ __launch_bounds__((RowE<N, 4>::NT), RowWaves<C>::value) void k_row_update(C* __restrict__ U, const C* __restrict__ ZA,
+                                                              const C* __restrict__ ZB, size_t V, double ss,
+                                                              int reunit, const C* __restrict__ tw, int next_noise,
+                                                              C* __restrict__ Xi, NoiseSrc ns,
+                                                              const double* __restrict__ sa_pos) {
+    __shared__ C lds[4 * padded<N, 4>::value];
+    row_update_body<N, BF, C>(lds, own_block(), U, ZA, ZB, V, ss, reunit, tw, next_noise, Xi, ns, sa_pos);
+}
+
+// ---------------------------------------------------------------------------
+// One launch per kernel class over every size group of a batch (row-major
+// intermediate planes).  Block b of a launch belongs to the group whose block
+// range holds it.  A group whose body needs NT threads, fewer than the T of
+// the launch, runs T / NT bodies side by side in the block (rows, or column
+// strips), each on its own thread range and its own part of LDS; all bodies
+// of one block have the same N, so the barriers inside them coincide.
+// ---------------------------------------------------------------------------
+template <typename C>
+struct GroupRun {
+    int N, nblk;                      // lattice size, blocks of this group in the launch
+    C *U, *Xi, *Vv, *ZA, *ZB;         // the group's trajectories, as for one grouped launch
+    size_t V;
+    NoiseSrc ns;                      // key of the group's first trajectory
+    const double *sa_pos, *sa_k, *q4, *sb;
+    const C* tw;
+};
+template <typename C>
+struct MultiRun {
+    int ng;
+    GroupRun<C> g[JW_MAX_GROUPS];
+};
+
+// the group of launch block b; b becomes the block's index inside the group
+template <typename C>
+__device__ __forceinline__ const GroupRun<C>& group_of(const MultiRun<C>& m, int& b) {
+    int g = 0;
+    while (g < m.ng - 1 && b >= m.g[g].nblk) {
+        b -= m.g[g].nblk;
+        g++;
+    }
+    return m.g[g];
+}
+
+template <int N>
+using int_c = std::integral_constant<int, N>;
+
+// f(int_c<N>, group, block index in the group) for the block's group, N from
+// the sizes a launch of T threads can hold (threads per body = MAXNT(N) <= T)
+template <int T, template <int> class NTOF, typename C, typename F>
+__device__ __forceinline__ void dispatch_group(const MultiRun<C>& m, F&& f) {
+    int b = blockIdx.x;
+    const GroupRun<C>& g = group_of(m, b);
+    switch (g.N) {
+        case 64: if constexpr (NTOF<64>::value <= T) f(int_c<64>{}, g, b); break;
+        case 128: if constexpr (NTOF<128>::value <= T) f(int_c<128>{}, g, b); break;
+        case 256: if constexpr (NTOF<256>::value <= T) f(int_c<256>{}, g, b); break;
+        case 512: if constexpr (NTOF<512>::value <= T) f(int_c<512>{}, g, b); break;
+        case 1024: if constexpr (NTOF<1024>::value <= T) f(int_c<1024>{}, g, b); break;
+        case 2048: if constexpr (NTOF<2048>::value <= T) f(int_c<2048>{}, g, b); break;
+        default: break;
+    }
+}
+template <int N> struct RowNT { static constexpr int value = RowE<N>::NT; };
+template <typename C>
+struct ColNT { template <int N> struct of { static constexpr int value = ColE<N, C, 1>::NT; }; };
+
+// LDS of a row launch of T threads: T / NT bodies of `planes` padded rows
+template <int T, int planes, typename C>
+struct RowMultiLds {
+    template <int N>
+    static constexpr int need() { return RowE<N>::NT <= T ? (T / RowE<N>::NT) * planes * padded<N, 4>::value : 0; }
+    static constexpr int a = need<64>() > need<128>() ? need<64>() : need<128>();
+    static constexpr int b = need<256>() > need<512>() ? need<256>() : need<512>();
+    static constexpr int c = need<1024>() > need<2048>() ? need<1024>() : need<2048>();
+    static constexpr int ab = a > b ? a : b;
+    static constexpr int value = ab > c ? ab : c;
+};
+template <typename C>
+struct ColMultiLds {
+    template <int N>
+    static constexpr int need() { return (1024 / ColE<N, C, 1>::NT) * padded<N, 3>::value * 2 * ColE<N, C, 1>::W; }
+    static constexpr int a = need<64>() > need<128>() ? need<64>() : need<128>();
+    static constexpr int b = need<256>() > need<512>() ? need<256>() : need<512>();
+    static constexpr int c = need<1024>() > need<2048>() ? need<1024>() : need<2048>();
+    static constexpr int ab = a > b ? a : b;
+    static constexpr int value = ab > c ? ab : c;
+};
+
+// the row (x), trajectory (z) and thread of a row body in block b of its group
+template <int N, int T>
+__device__ __forceinline__ BlockIx row_slot(int b) {
+    constexpr int NT = RowE<N>::NT, RPB = T / NT, NRB = N / RPB;
+    const int s = threadIdx.x / NT;
+    return BlockIx{(b % NRB) * RPB + s, 0, b / NRB, (int)threadIdx.x % NT};
+}
+template <int N, int T>
+__device__ __forceinline__ int row_sub() { return threadIdx.x / RowE<N>::NT; }
+
+template <typename C, int T>
+__global__ __launch_bounds__(T, (NoiseWaves<64, C>::value)) void k_row_noise_fwd_multi(MultiRun<C> m) {
+    __shared__ C lds[RowMultiLds<T, 2, C>::value];
+    dispatch_group<T, RowNT>(m, [&](auto n, const GroupRun<C>& g, int b) {
+        constexpr int N = decltype(n)::value;
+        row_noise_fwd_body<N, 0, C>(lds + row_sub<N, T>() * 2 * padded<N, 4>::value, row_slot<N, T>(b), g.Xi, g.V,
+                                    g.ns, g.sa_pos, g.tw);
+    });
+}
+template <typename C, int T>
+__global__ __launch_bounds__(T, RowWaves<C>::value) void k_row_uxiu_fwd_multi(MultiRun<C> m) {
+    __shared__ C lds[RowMultiLds<T, 4, C>::value];
+    dispatch_group<T, RowNT>(m, [&](auto n, const GroupRun<C>& g, int b) {
+        constexpr int N = decltype(n)::value;
+        row_uxiu_fwd_body<N, 0, C>(lds + row_sub<N, T>() * 4 * padded<N, 4>::value, row_slot<N, T>(b), g.Vv, g.U, g.V,
+                                   g.ns, g.sa_pos, g.tw);
+    });
+}
+template <typename C, int T>
+__global__ __launch_bounds__(T, RowWaves<C>::value) void k_row_uxiu_corr_multi(MultiRun<C> m) {
+    __shared__ C lds[RowMultiLds<T, 4, C>::value];
+    dispatch_group<T, RowNT>(m, [&](auto n, const GroupRun<C>& g, int b) {
+        constexpr int N = decltype(n)::value;
+        row_uxiu_corr_body<N, 0, C>(lds + row_sub<N, T>() * 4 * padded<N, 4>::value, row_slot<N, T>(b), g.Vv, g.U, g.Xi,
+                                    g.V, g.tw);
+    });
+}
+template <typename C, int T>
+__global__ __launch_bounds__(T, RowWaves<C>::value) void k_row_update_multi(MultiRun<C> m, double ss, int reunit) {
+    __shared__ C lds[RowMultiLds<T, 4, C>::value];
+    dispatch_group<T, RowNT>(m, [&](auto n, const GroupRun<C>& g, int b) {
+        constexpr int N = decltype(n)::value;
+        row_update_body<N, 0, C>(lds + row_sub<N, T>() * 4 * padded<N, 4>::value, row_slot<N, T>(b), g.U, g.ZA, g.ZB,
+                                 g.V, ss, reunit, g.tw, 0, g.Xi, g.ns, g.sa_pos);
+    });
+}
+// column passes: T = 1024; block b of a group: strip block, plane pair, trajectory
+// in that order (the order of the grouped launch's grid)
+template <typename C, int CORR, int GEN>
+__global__ __launch_bounds__(1024, 1) void k_col_kmult_multi(MultiRun<C> m, int use_v, int sqrt_coupling,
+                                                             int noise_scale) {
+    __shared__ C lds[ColMultiLds<C>::value];
+    dispatch_group<1024, ColNT<C>::template of>(m, [&](auto n, const GroupRun<C>& g, int b) {
+        constexpr int N = decltype(n)::value;
+        typedef ColE<N, C, CORR> CE;
+        constexpr int SPB = 1024 / CE::NT, NSB = N / CE::W / SPB;
+        const int s = threadIdx.x / CE::NT;
+        const BlockIx bi{(b % NSB) * SPB + s, (b / NSB) % 4, b / (NSB * 4), (int)threadIdx.x % CE::NT};
+        C* X = use_v ? g.Vv : g.Xi;
+        col_kmult_body<N, CORR, C, 0, 0, GEN>(lds + s * padded<N, 3>::value * 2 * CE::W, bi, use_v ? g.ZB : g.ZA, X,
+                                              X + 4 * g.V, g.V, sqrt_coupling, g.q4, g.sb, g.sa_k, g.tw, g.ns,
+                                              noise_scale);
+    });
+}
+
+// blocks of group g in a row launch of T threads / in a column launch
+template <typename C>
+inline int row_blocks(int N, int count, int T) {
+    const int nt = N / 4 < 64 ? 64 : (N / 4 > 1024 ? 1024 : N / 4);      // RowE<N>::NT
+    return count * N / (T / nt);
+}
+template <typename C>
+inline int col_blocks(int N, int count) {
+    const int W = 4096 / N > 16 ? 16 : 4096 / N;                          // ColE<N>::W
+    const int elems = 2 * W * N, nt = elems / 8 < 64 ? 64 : (elems / 8 > 1024 ? 1024 : elems / 8);
+    return count * 4 * (N / W) / (1024 / nt);
+}
+
+// one step of every group of a batch, one launch per kernel class (row-major
+// planes; the noise pass separate from the update).  counts[g]: trajectories.
+template <typename C>
+static hipError_t step_multi_t(hipStream_t st, MultiRun<C> m, const int* counts, int maxN, const StepArgs& a) {
+    if (m.ng < 1 || m.ng > JW_MAX_GROUPS) return hipErrorInvalidValue;
+    const int T = maxN > 1024 ? 512 : 256;
+    int nrow = 0, ncol = 0;
+    for (int g = 0; g < m.ng; g++) {
+        if (m.g[g].N < 64 || m.g[g].N > 2048) return hipErrorInvalidValue;
+        nrow += row_blocks<C>(m.g[g].N, counts[g], T);
+        ncol += col_blocks<C>(m.g[g].N, counts[g]);
+    }
+    auto set_blocks = [&](bool col) {
+        for (int g = 0; g < m.ng; g++)
+            m.g[g].nblk = col ? col_blocks<C>(m.g[g].N, counts[g]) : row_blocks<C>(m.g[g].N, counts[g], T);
+    };
+    const int wb = a.corr || a.kspace;
+    hipError_t e;
+    if (!a.kspace) {
+        set_blocks(false);
+        if (T == 256) hipLaunchKernelGGL((k_row_noise_fwd_multi<C, 256>), dim3(nrow), dim3(256), 0, st, m);
+        else hipLaunchKernelGGL((k_row_noise_fwd_multi<C, 512>), dim3(nrow), dim3(512), 0, st, m);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    set_blocks(true);
+    if (a.kspace)
+        hipLaunchKernelGGL((k_col_kmult_multi<C, 1, 1>), dim3(ncol), dim3(1024), 0, st, m, 0, a.sqrt_coupling, a.corr);
+    else if (a.corr)
+        hipLaunchKernelGGL((k_col_kmult_multi<C, 1, 0>), dim3(ncol), dim3(1024), 0, st, m, 0, a.sqrt_coupling, 1);
+    else
+        hipLaunchKernelGGL((k_col_kmult_multi<C, 0, 0>), dim3(ncol), dim3(1024), 0, st, m, 0, a.sqrt_coupling, 0);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    set_blocks(false);
+    if (wb) {
+        if (T == 256) hipLaunchKernelGGL((k_row_uxiu_corr_multi<C, 256>), dim3(nrow), dim3(256), 0, st, m);
+        else hipLaunchKernelGGL((k_row_uxiu_corr_multi<C, 512>), dim3(nrow), dim3(512), 0, st, m);
+    } else {
+        if (T == 256) hipLaunchKernelGGL((k_row_uxiu_fwd_multi<C, 256>), dim3(nrow), dim3(256), 0, st, m);
+        else hipLaunchKernelGGL((k_row_uxiu_fwd_multi<C, 512>), dim3(nrow), dim3(512), 0, st, m);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    set_blocks(true);
+    hipLaunchKernelGGL((k_col_kmult_multi<C, 0, 0>), dim3(ncol), dim3(1024), 0, st, m, 1, a.sqrt_coupling, 0);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    set_blocks(false);
+    if (T == 256)
+        hipLaunchKernelGGL((k_row_update_multi<C, 256>), dim3(nrow), dim3(256), 0, st, m, a.sqrt_step, a.reunit);
+    else
+        hipLaunchKernelGGL((k_row_update_multi<C, 512>), dim3(nrow), dim3(512), 0, st, m, a.sqrt_step, a.reunit);
+    return hipGetLastError();
+}
+
+// the batch's groups as one MultiRun (at most JW_MAX_GROUPS)
+template <typename C>
+static hipError_t step_multi_from(hipStream_t st, const GroupArgs* ga, int ng, const StepArgs& a) {
+    if (ng < 1 || ng > JW_MAX_GROUPS) return hipErrorInvalidValue;
+    MultiRun<C> m{};
+    int counts[JW_MAX_GROUPS], maxN = 0;
+    m.ng = ng;
+    for (int g = 0; g < ng; g++) {
+        const GroupArgs& s = ga[g];
+        m.g[g] = GroupRun<C>{s.N, 0, (C*)s.U, (C*)s.Xi, (C*)s.Vv, (C*)s.ZA, (C*)s.ZB, s.V, s.ns,
+                             s.sa_pos, s.sa_k, s.q4, s.sb, (const C*)s.tw};
+        counts[g] = s.count;
+        maxN = s.N > maxN ? s.N : maxN;
+    }
+    return step_multi_t<C>(st, m, counts, maxN, a);
 }
 
 // ---------------------------------------------------------------------------

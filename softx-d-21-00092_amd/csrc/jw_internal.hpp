@@ -144,6 +144,20 @@ struct StepArgs {
 };
 hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a);
 hipError_t step_fused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a);
+// one step of every size group of a batch with one launch per kernel class
+// (row-major intermediate planes, at most 8 groups of 64 <= N <= 2048); the
+// pointers are the group's state arrays in the element type of the form
+#define JW_MAX_GROUPS 8
+struct GroupArgs {
+    int N, count;
+    void *U, *Xi, *Vv, *ZA, *ZB;
+    size_t V;
+    NoiseSrc ns;
+    const double *sa_pos, *sa_k, *q4, *sb;
+    const void* tw;
+};
+hipError_t step_multi(hipStream_t st, const GroupArgs* g, int ng, const StepArgs& a);
+hipError_t step_multi_f32(hipStream_t st, const GroupArgs* g, int ng, const StepArgs& a);
 // N = 4096, 8192: fused column passes, separate row transforms and site passes
 bool large_supported(int N);
 hipError_t step_large(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a);

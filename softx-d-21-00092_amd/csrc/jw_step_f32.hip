@@ -10,6 +10,10 @@ hipError_t step_fused_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, 
     return step_fused_t(c, U, Xi, Vv, ZA, ZB, a);
 }
 
+hipError_t step_multi_f32(hipStream_t st, const GroupArgs* g, int ng, const StepArgs& a) {
+    return step_multi_from<cplxf>(st, g, ng, a);
+}
+
 hipError_t step_large_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a) {
     return step_large_t(c, U, Xi, Vv, ZA, ZB, a);
 }

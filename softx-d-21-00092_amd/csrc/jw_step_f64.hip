@@ -14,6 +14,10 @@ hipError_t step_fused(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB
     return step_fused_t(c, U, Xi, Vv, ZA, ZB, a);
 }
 
+hipError_t step_multi(hipStream_t st, const GroupArgs* g, int ng, const StepArgs& a) {
+    return step_multi_from<cplx>(st, g, ng, a);
+}
+
 hipError_t step_large(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a) {
     return step_large_t(c, U, Xi, Vv, ZA, ZB, a);
 }

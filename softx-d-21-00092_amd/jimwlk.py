@@ -109,6 +109,8 @@ def lib():
         L.jw_batch_set_layout.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.jw_batch_layout.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.jw_batch_fft_mode.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_batch_set_launch.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.jw_batch_launch.argtypes = [ctypes.c_void_p]
         L.jw_batch_evolve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                       ctypes.POINTER(Alpha), ctypes.c_int, ctypes.c_int]
         L.jw_batch_sync.argtypes = [ctypes.c_void_p]
@@ -387,6 +389,14 @@ class Batch:
 
     def layout(self, group):
         return lib().jw_batch_layout(self.h, group)
+
+    def set_launch(self, mode):
+        """0: per-group launches on their own streams; 1: one launch per kernel class over
+        all groups; -1: timed on the next evolution."""
+        _check(lib().jw_batch_set_launch(self.h, int(mode)))
+
+    def launch(self):
+        return lib().jw_batch_launch(self.h)
 
     def fft_mode(self, group):
         """1 when the tuner put the group's FP32 radix-16 column stages on the matrix cores."""

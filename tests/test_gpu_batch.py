@@ -183,3 +183,73 @@ def test_batch_rejects(jw):
         b.set(1, c.field())                    # no such trajectory
     b.close()
     c.close()
+
+
+@pytest.mark.parametrize("fp32", [1, 0])
+@pytest.mark.parametrize("kspace", [0, 1])
+def test_batch_one_launch_matches_single(jw, fp32, kspace):
+    """One launch per kernel class over every size group (jw_batch_set_launch(1)):
+    the same arithmetic per trajectory as a single lattice with row-major planes, so
+    every trajectory agrees with it to 1e-13, with several rows or column strips of
+    the small groups in one workgroup (64^2, 128^2 beside 256^2 and 2048^2)."""
+    import math
+    groups = [(64, 3), (128, 2), (256, 1), (2048, 1)]
+    print(f"batch: one launch per kernel class, fp32={fp32}, momentum noise={kspace}", flush=True)
+    ctxs, states = mv_states(jw, groups, Ny=2)
+    b = jw.Batch(groups, fp32=fp32)
+    b.set_launch(1)
+    assert b.launch() == 1
+    if kspace:
+        b.set_noise_space(jw.NOISE_MOMENTUM)
+    for t, (N, U0) in enumerate(states):
+        b.set(t, ctxs[N].field(9, U0))
+    b.philox(5, 9, 0)
+    coupling, ds = (4, 5e-5) if kspace else (3, 1e-4)
+    al = lambda N: jw.alpha(lambda_L=0.02 * N, mu0_over_lambda=math.exp(2 * math.pi / (9 * 0.7))) if coupling == 4 \
+        else None
+    # the batch passes one alpha; each group's coupling table is built for its own N from it
+    b.evolve(coupling, 1, ds, al(64), nsteps=12, reunit_every=10)
+    b.sync()
+    for t, (N, U0) in enumerate(states):
+        c = ctxs[N]
+        if kspace:
+            c.set_noise_space(jw.NOISE_MOMENTUM)
+        got = c.field()
+        b.get(t, got)
+        ref = single(jw, c, U0, fp32, 5, 9 + t, 0, 12, coupling=coupling, ds=ds, al=al(64))
+        c.set_noise_space(jw.NOISE_POSITION)
+        assert frob_max(got.download(), ref) <= 1e-13, (t, N)
+        assert frob_max(ref, U0) > 1e-4
+    b.close()
+    for c in ctxs.values():
+        c.close()
+
+
+def test_batch_launch_tuning(jw):
+    """The launch mode is timed on the first evolution; either choice evolves every
+    trajectory as a single lattice does (FP32 rounding apart from the layouts)."""
+    groups = [(256, 4), (512, 2)]
+    ctxs, states = mv_states(jw, groups, Ny=2)
+    b = jw.Batch(groups, fp32=True)
+    assert b.launch() == -1
+    for t, (N, U0) in enumerate(states):
+        b.set(t, ctxs[N].field(9, U0))
+    b.philox(3, 0, 0)
+    b.evolve(3, 1, 1e-4, None, nsteps=10, reunit_every=10)
+    b.sync()
+    print(f"batch: tuned launch mode {b.launch()}, layouts {[b.layout(g) for g in range(2)]}", flush=True)
+    assert b.launch() in (0, 1)
+    gi = [g for g, (N, count) in enumerate(groups) for _ in range(count)]
+    for t, (N, U0) in enumerate(states):
+        c = ctxs[N]
+        got = c.field()
+        b.get(t, got)
+        c.set_mixed_fft(b.fft_mode(gi[t]))
+        ref = single(jw, c, U0, 1, 3, t, 0, 10)
+        c.set_mixed_fft(0)
+        assert frob_max(got.download(), ref) <= 2e-6, t
+    with pytest.raises(jw.JwError):
+        b.set_launch(2)
+    b.close()
+    for c in ctxs.values():
+        c.close()
