@@ -307,8 +307,9 @@ __device__ __forceinline__ void box_muller_pair(uint32_t site, uint32_t step, ui
 // The same pair in FP32 from the same Philox words (mixed-precision form):
 // log u0 from the 53-bit integer (as log1p of -(1 - u0) above 1/2, so that
 // small |z| keep their relative precision) and the angle as pi x with
-// x = 2 u1 - 1 in (-1, 1): sincospi(x + 1) = -sincospi(x).  Each normal agrees
-// with the FP64 pair to a few FP32 ulps of max(|z|, 1).
+// x = 2 u1 - 1 in (-1, 1): sincospi(x + 1) = -sincospi(x), on the hardware
+// sine and cosine.  Each normal agrees with the FP64 pair to a few FP32 ulps
+// of max(|z|, 1).
 __device__ __forceinline__ void box_muller_pair_f(uint32_t site, uint32_t step, uint32_t tag, uint32_t call,
                                                   uint32_t seed, uint32_t traj, float& z0, float& z1) {
     uint32_t c[4] = {site, step, tag, call};
@@ -327,10 +328,12 @@ __device__ __forceinline__ void box_muller_pair_f(uint32_t site, uint32_t step, 
     if (!lo) lu = a == 1.0f ? -t : lu * (t * __builtin_amdgcn_rcpf(1.0f - a));
     const float rr = sqrtf(-2.0f * lu);
     const int64_t d = 2 * (int64_t)m1 + 1 - (1ll << 53);            // x = d 2^-53
-    float s, co;
-    sincospif((float)(double)d * 0x1.0p-53f, &s, &co);
-    z0 = -rr * co;
-    z1 = -rr * s;
+    // sin and cos of pi x on the hardware's revolution-based units (x / 2
+    // revolutions, in (-1/2, 1/2)): within 1.9e-7 of FP64 sincospi on all inputs,
+    // against 1.2e-7 for the library's sincospif, in far fewer instructions
+    const float rev = (float)(double)d * 0x1.0p-54f;
+    z0 = -rr * __builtin_amdgcn_cosf(rev);
+    z1 = -rr * __builtin_amdgcn_sinf(rev);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,10 +367,9 @@ __device__ __forceinline__ void box_muller32(uint32_t w0, uint32_t w1, float& z0
     if (!lo) lu = a == 1.0f ? -t : lu * (t * __builtin_amdgcn_rcpf(1.0f - a));
     const float rr = sqrtf(-2.0f * lu);
     const int64_t d = 2 * (int64_t)w1 + 1 - (1ll << 32);            // x = d 2^-32
-    float s, co;
-    sincospif((float)(double)d * 0x1.0p-32f, &s, &co);
-    z0 = -rr * co;
-    z1 = -rr * s;
+    const float rev = (float)(double)d * 0x1.0p-33f;                 // x / 2 revolutions, as above
+    z0 = -rr * __builtin_amdgcn_cosf(rev);
+    z1 = -rr * __builtin_amdgcn_sinf(rev);
 }
 template <typename T>
 __device__ __forceinline__ void kspace_normals4(uint32_t k, uint32_t step, uint32_t tag, uint32_t p, uint32_t seed,
