@@ -68,20 +68,6 @@ __device__ __forceinline__ int xcd_strip(int b, int nstrips) {
     if (nstrips % 8) return b;
     return (b % 8) * (nstrips / 8) + b / 8;
 }
-// strip and plane pair of column-pass block b (a grid of 4 nstrips blocks): the
-// four pairs of a strip run on one XCD in consecutive turns, so that the
-// sqrt(alpha) entries they all read come from that XCD's L2 after the first,
-// and neighbouring strips (sharing cache lines) on one XCD at neighbouring times
-__device__ __forceinline__ void strip_pair(int b, int nstrips, int& strip, int& p) {
-    if (nstrips % 8) {
-        strip = b >> 2;
-        p = b & 3;
-        return;
-    }
-    const int t = b >> 3;
-    strip = (b & 7) * (nstrips >> 3) + (t >> 2);
-    p = t & 3;
-}
 
 // offset of site (x, y) in an intermediate plane of layout BF (0 and 2
 // row-major, 1 row-major for the 64-byte-strip column pass, 4 and 8 row blocks)
@@ -450,9 +436,9 @@ __global__ __launch_bounds__(RowE<N>::NT, RowWaves<C>::value) void k_row_uxiu_co
 //   by 1 when noise_scale is 0) and F_col^-1[sa X^ / V] is written back over X
 //   (likewise Y).
 //   GEN (with CORR): X^, Y^ are not transformed from X, Y but drawn per mode
-//   (kspace_normals4, plane pair p, noise key from ns); X and Y
+//   (kspace_normals4, plane pair p = blockIdx.y, noise key from ns); X and Y
 //   then receive the row spectra of the position-space noise.
-// X, Y: row spectra of plane p (block index: strip_pair); strips of W columns.  PI: X is
+// X, Y: row spectra of plane p = blockIdx.y; strips of W columns.  PI: X is
 // the pair-interleaved array (layout 2) holding both directions, Y unused.
 // LDS row x holds [X(x, c0..c0+W-1), Y(x, c0..c0+W-1)]: 2W sequences of length N
 // with element stride 2W.
@@ -474,7 +460,7 @@ __device__ __forceinline__ void col_kmult_body(C* lds, BlockIx bi, C* __restrict
     constexpr int S2 = 2 * W;
     const int tid = bi.tid;
     const auto twl = tw_source<N, sizeof(C) == sizeof(float2)>(tw);
-    const int strip = bi.bx;
+    const int strip = xcd_strip(bi.bx, N / W);
     const int col0 = strip * W;
     Z += zofs_z(bi.bz, 4, V);
     X += zofs_z(bi.bz, 8, V);
@@ -591,10 +577,7 @@ __global__ __launch_bounds__((ColE<N, C, CORR>::NT), (MF && !CORR ? 2 * ColE<N, 
                                                           const double* __restrict__ sa,
                                                           const C* __restrict__ tw, NoiseSrc ns, int noise_scale) {
     __shared__ C lds[padded<N, 3>::value * 2 * ColE<N, C, CORR>::W];
-    int strip, p;
-    strip_pair(blockIdx.x, N / ColE<N, C, CORR>::W, strip, p);
-    col_kmult_body<N, CORR, C, PI, MF, GEN>(lds, BlockIx{strip, p, (int)blockIdx.z, (int)threadIdx.x}, Z, X, Y, V,
-                                            sqrt_coupling, q4, sb, sa, tw, ns, noise_scale);
+    col_kmult_body<N, CORR, C, PI, MF, GEN>(lds, own_block(), Z, X, Y, V, sqrt_coupling, q4, sb, sa, tw, ns, noise_scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -652,11 +635,8 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
     Z += zofs(4, V);
     X += zofs(8, V);
     Y += zofs(8, V);
-    // segment strip and plane pair (the four pairs of a strip, and the strips of
-    // one 128-byte line, on one XCD at neighbouring times)
-    int strip, pair;
-    strip_pair(blockIdx.x, N / CPB, strip, pair);
-    const int y0 = strip * CPB;
+    // 32-byte strips: the four strips of one 128-byte line on one XCD at neighbouring times
+    const int y0 = (BF >= 16 ? xcd_strip(blockIdx.x, N / CPB) : (int)blockIdx.x) * CPB;
     const double invV = 1.0 / ((double)N * (double)N);
     // global offset within a plane / LDS slot of loaded element e: x-block
     // e / SEG, slot e mod SEG = c BF + x mod BF (row-major, BF = 1: row e / SEG,
@@ -710,7 +690,7 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
             if constexpr (GEN) {
                 if (d == 0) {
                     T g[4];
-                    kspace_normals4((uint32_t)((size_t)kx * N + ky), nk.step, nk.tag, pair, nk.seed, nk.traj, g);
+                    kspace_normals4((uint32_t)((size_t)kx * N + ky), nk.step, nk.tag, blockIdx.y, nk.seed, nk.traj, g);
                     v = mkc<C>(g[0] * (T)N, g[1] * (T)N);      // sqrt(V) = N
                     yg[q] = mkc<C>(g[2] * (T)N, g[3] * (T)N);
                 } else {
@@ -738,9 +718,9 @@ __global__ __launch_bounds__((ColB<N, BF, C>::NT), (ColB<N, BF, C>::WAVES)) void
             __syncthreads();
         }
     };
-    // plane pair `pair` of this block (one pass over the pairs would let the compiler
+    // plane pair p = blockIdx.y (one pass over the pairs would let the compiler
     // hoist every twiddle and kernel factor out of the loop into registers)
-    const size_t po = (size_t)pair * V;
+    const size_t po = (size_t)blockIdx.y * V;
     C acc[PER];
     direction(X + po, acc, 0);
     direction(Y + po, acc, 1);
@@ -1116,7 +1096,7 @@ static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int 
 #define JW_COL_ARGS Z, X, Y, V, sqrt_coupling, c->q4, c->sb, sa_k, tw, ns, noise_scale
     if constexpr (BF == 0 || BF == 2) {
         constexpr int W = ColE<N, C>::W, PI = BF == 2;
-        const dim3 grid(4 * (N / W), 1, nbatch);
+        const dim3 grid(N / W, 4, nbatch);
         // split-BF16 matrix-core radix-16 stages: FP32 storage, row-major layout
         if constexpr (sizeof(C) == sizeof(float2) && BF == 0) {
             if (c->fft_mfma) {
@@ -1140,7 +1120,7 @@ static hipError_t col_pass(jw_ctx* c, C* Z, C* X, C* Y, const double* sa_k, int 
             hipLaunchKernelGGL((k_col_kmult<N, 0, C, PI>), grid, dim3(ColE<N, C, 0>::NT), 0, c->stream, JW_COL_ARGS);
     } else {
         constexpr int CPB = ColB<N, BF, C>::CPB, NT = ColB<N, BF, C>::NT;
-        const dim3 grid(4 * (N / CPB), 1, nbatch);
+        const dim3 grid(N / CPB, 4, nbatch);
         // the matrix-core stages in the 16-byte strips of the large lattices
         if constexpr (sizeof(C) == sizeof(float2) && BF == 32) {
             if (c->fft_mfma) {
