@@ -74,6 +74,8 @@ PLANES = {"correlated": {"row_noise_fwd": 8, "col_kmult": (20, 12), "row_uxiu_fw
 # measured HBM traffic per launch of the dominant kernels (rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a plain copy)
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_N2048.json")
+# matrix-core use of the split-BF16 column stages (rocprofv3 --pmc, mixed_mfma_k)
+MFMA_FILE = os.path.join(REPO, "profiles", "mfma_N2048.json")
 
 
 def parse():
@@ -95,7 +97,7 @@ def parse():
     p.add_argument("--cpu-steps-omp", type=int, default=2, help="CPU baseline steps with every loop threaded")
     p.add_argument("--fused-mode", type=int, default=1,
                    help="jw_set_fused mode: 1 default, 2/3/4 row-major/4-row/8-row blocked intermediates")
-    p.add_argument("--forms", default="fp64,mixed,fp64_k,mixed_k",
+    p.add_argument("--forms", default="fp64,mixed,fp64_k,mixed_k,mixed_mfma_k",
                    help="forms to time (profiling runs may pick one); *_k: momentum-space noise")
     p.add_argument("--noise-space", default="momentum", choices=["position", "momentum"],
                    help="noise of the headline forms, the batched form and the S(r) comparison")
@@ -346,6 +348,15 @@ def main():
               f"{large['Qs_rel_err_vs_fp64']}", file=sys.stderr, flush=True)
         cl.close()
 
+    mfma = None
+    if "mixed_mfma_k" in res:
+        mfma = {"form": "mixed_mfma_k: radix-16 column stages as split-BF16 products on v_mfma_f32_32x32x16_bf16",
+                "steps_per_s": res["mixed_mfma_k"]["steps_per_s"],
+                "column_ms_per_step": res["mixed_mfma_k"]["kernels"].get("col_kmult", {}).get("ms_per_step")}
+        if os.path.exists(MFMA_FILE):
+            with open(MFMA_FILE) as f:
+                mfma["profile"] = json.load(f)
+
     # measured traffic of the dominant kernels, when a profile summary is present
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
@@ -367,12 +378,20 @@ def main():
             run(form, F, a.sr_steps)
             S[form] = ctx.dipole_from_corr(ctx.measure_corr(F))
             F.close()
+        if "mixed_mfma_k" in forms and ks:       # the matrix-core form on the same stream
+            F = ctx.field(9, U0)
+            ctx.philox(3, 0, 0)
+            run("mixed_mfma_k", F, a.sr_steps)
+            S["mixed_mfma_k"] = ctx.dipole_from_corr(ctx.measure_corr(F))
+            F.close()
         S0 = ctx.dipole_from_corr(ctx.measure_corr(ctx.field(9, U0)))
         r, s64, cnt = jw.radial_bins(S[f64], N, N // 2)
         _, s32, _ = jw.radial_bins(S[fmix], N, N // 2)
         sr = {"steps": a.sr_steps, "noise_space": a.noise_space, "max_abs_err": float(np.abs(S[f64] - S[fmix]).max()),
               "max_abs_err_radial": float(np.abs(s64 - s32)[cnt > 0].max()),
               "max_abs_change_from_start": float(np.abs(S[f64] - S0).max()), "bound": 2e-5}
+        if "mixed_mfma_k" in S:
+            sr["max_abs_err_mfma"] = float(np.abs(S[f64] - S["mixed_mfma_k"]).max())
         print(f"bench: S(r) after {a.sr_steps} steps: max |mixed - fp64| = {sr['max_abs_err']:.3e} "
               f"(evolution moved S by {sr['max_abs_change_from_start']:.3e})", file=sys.stderr, flush=True)
 
@@ -448,6 +467,7 @@ def main():
            if fmix in res and f64 in res else None,
            "ensemble": ensemble,
            "large": large,
+           "mfma": mfma,
            "sr_err_vs_fp64": sr,
            "roofline": h["roofline"], "step_roofline": h["step_roofline"], "kernels": h["kernels"],
            "roofline_fp64": res[f64]["roofline"] if f64 in res else None,
