@@ -294,6 +294,8 @@ struct jw_batch {
 namespace {
 
 size_t elem_size(const jw_batch* b) { return b->fp32 ? sizeof(jw::cplxf) : sizeof(cplx); }
+// U planes per trajectory: the mixed form's fused passes hold rows 0 and 1
+int u_planes(const jw_batch* b) { return b->fp32 ? 6 : 9; }
 
 void batch_free(jw_batch* b) {
     if (!b) return;
@@ -427,7 +429,7 @@ int batch_tune_launch(jw_batch* b, int coupling, double step, const jw::Alpha& a
     int rc = JW_OK;
     for (auto& g : b->g) HIP_CHECK(hipStreamSynchronize(g.ctx->stream));
     for (int gi = 0; gi < ng && rc == JW_OK; gi++) {
-        const size_t bytes = (size_t)b->g[gi].count * 9 * b->g[gi].ctx->V * elem_size(b);
+        const size_t bytes = (size_t)b->g[gi].count * u_planes(b) * b->g[gi].ctx->V * elem_size(b);
         if (hipMalloc(&uc[gi], bytes) != hipSuccess ||
             hipMemcpyAsync(uc[gi], b->g[gi].U, bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
             rc = fail(JW_ERR_ALLOC, "launch tuning buffer allocation failed");
@@ -479,7 +481,7 @@ int batch_tune(jw_batch* b, int coupling, double step, const jw::Alpha& al) {
     for (size_t gi = 0; gi < b->g.size(); gi++) {
         BatchGroup& g = b->g[gi];
         if (g.layout >= 0) continue;
-        const size_t ubytes = (size_t)g.count * 9 * g.ctx->V * elem_size(b);
+        const size_t ubytes = (size_t)g.count * u_planes(b) * g.ctx->V * elem_size(b);
         void* ucopy = nullptr;
         if (hipMalloc(&ucopy, ubytes) != hipSuccess) return fail(JW_ERR_ALLOC, "tuning buffer allocation failed");
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1069,7 +1071,9 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
         rc = ensure_sa16(c);
         sa_k = c->sa16;
     }
-    if (rc == JW_OK && jw::u_convert(c, U32, U->d, 1) != hipSuccess) rc = fail(JW_ERR_HIP, "U conversion failed");
+    // the fused passes hold rows 0 and 1 of U (row 2 rebuilt), the others all nine entries
+    const int uplanes = fused ? 6 : 9;
+    if (rc == JW_OK && jw::u_convert(c, U32, U->d, 1, uplanes) != hipSuccess) rc = fail(JW_ERR_HIP, "U conversion failed");
     for (int it = 0; rc == JW_OK && it < nsteps; it++) {
         bool ok;
         jw::NoiseSrc ns = take_noise(c, 16, c->noise_kspace ? JW_TAG_KSPACE : JW_TAG_LANGEVIN, &ok);
@@ -1089,7 +1093,7 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
                                                            sqrt(step), reunit);
         if (he != hipSuccess) rc = fail(JW_ERR_HIP, "%s", hipGetErrorString(he));
     }
-    if (rc == JW_OK && jw::u_convert(c, U32, U->d, 0) != hipSuccess) rc = fail(JW_ERR_HIP, "U conversion failed");
+    if (rc == JW_OK && jw::u_convert(c, U32, U->d, 0, uplanes) != hipSuccess) rc = fail(JW_ERR_HIP, "U conversion failed");
     c->layout = saved_layout;
     return rc;
 }
@@ -1298,10 +1302,10 @@ int jw_batch_create(jw_batch** out, int device, int ngroups, const int* N, const
         if ((rc = jw_create(&g.ctx, N[i], device))) break;
         g.ctx->layout = -1;
         const size_t V = g.ctx->V, n = (size_t)g.count * V * es;
-        if (hipMalloc(&g.U, 9 * n) != hipSuccess || hipMalloc(&g.Xi, 8 * n) != hipSuccess ||
+        if (hipMalloc(&g.U, u_planes(b) * n) != hipSuccess || hipMalloc(&g.Xi, 8 * n) != hipSuccess ||
             hipMalloc(&g.Vv, 8 * n) != hipSuccess || hipMalloc(&g.ZA, 4 * n) != hipSuccess ||
             hipMalloc(&g.ZB, 4 * n) != hipSuccess) {
-            rc = fail(JW_ERR_ALLOC, "batch group %d: %zu bytes of state", i, 33 * n);
+            rc = fail(JW_ERR_ALLOC, "batch group %d: %zu bytes of state", i, (24 + u_planes(b)) * n);
             break;
         }
         if (hipEventCreateWithFlags(&g.join, hipEventDisableTiming) != hipSuccess) rc = fail(JW_ERR_HIP, "event");
@@ -1329,7 +1333,7 @@ int jw_batch_set(jw_batch* b, int t, const jw_field* U) {
     HIP_CHECK(hipStreamSynchronize(U->ctx->stream));     // pending work on the source field
     HIP_CHECK(hipStreamSynchronize(b->stream));           // and on the batch
     if (b->fp32) {
-        HIP_CHECK(jw::u_convert(g->ctx, (jw::cplxf*)g->U + (size_t)k * 9 * V, U->d, 1));
+        HIP_CHECK(jw::u_convert(g->ctx, (jw::cplxf*)g->U + (size_t)k * 6 * V, U->d, 1, 6));
     } else {
         HIP_CHECK(hipMemcpyAsync((cplx*)g->U + (size_t)k * 9 * V, U->d, 9 * V * sizeof(cplx),
                                  hipMemcpyDeviceToDevice, g->ctx->stream));
@@ -1348,7 +1352,7 @@ int jw_batch_get(jw_batch* b, int t, jw_field* U) {
     HIP_CHECK(hipStreamSynchronize(b->stream));
     HIP_CHECK(hipStreamSynchronize(U->ctx->stream));     // pending work on the destination field
     if (b->fp32) {
-        HIP_CHECK(jw::u_convert(g->ctx, (jw::cplxf*)g->U + (size_t)k * 9 * V, U->d, 0));
+        HIP_CHECK(jw::u_convert(g->ctx, (jw::cplxf*)g->U + (size_t)k * 6 * V, U->d, 0, 6));
     } else {
         HIP_CHECK(hipMemcpyAsync(U->d, (cplx*)g->U + (size_t)k * 9 * V, 9 * V * sizeof(cplx),
                                  hipMemcpyDeviceToDevice, g->ctx->stream));

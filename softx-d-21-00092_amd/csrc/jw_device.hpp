@@ -38,6 +38,7 @@ __device__ __forceinline__ cplxf cscale(cplxf a, float s) { return mkf(a.x * s, 
 __device__ __forceinline__ cplxf cmuli(cplxf a) { return mkf(-a.y, a.x); }
 __device__ __forceinline__ cplxf cmulmi(cplxf a) { return mkf(a.y, -a.x); }
 __device__ __forceinline__ cplxf cmulc(cplxf a, cplxf b) { return mkf(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
+__device__ __forceinline__ cplxf conjg(cplxf a) { return mkf(a.x, -a.y); }
 
 // complex value of type C (double2 or float2) from two doubles, and back
 template <typename C>

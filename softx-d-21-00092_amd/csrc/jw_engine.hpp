@@ -136,6 +136,29 @@ struct site_mat { typedef m3 type; };
 template <>
 struct site_mat<float2> { typedef m3f type; };
 
+// U planes held per trajectory by the fused passes: FP64 all nine; the mixed
+// form rows 0 and 1 (six planes), row 2 rebuilt on load as conj(row 0 x row 1)
+// -- U is in SU(3) to FP32 rounding and projected every few steps, and the
+// projection rebuilds row 2 the same way -- so that the row passes move a third
+// less U.
+template <typename C>
+struct UPlanes { static constexpr int value = sizeof(C) == sizeof(float2) ? 6 : 9; };
+template <typename C, typename M>
+__device__ __forceinline__ void load_u(M& u, const C* __restrict__ U, size_t V, size_t i) {
+#pragma unroll
+    for (int k = 0; k < UPlanes<C>::value; k++) u.m[k] = U[k * V + i];
+    if constexpr (UPlanes<C>::value == 6) {
+        u.m[6] = conjg(csub(cmul(u.m[1], u.m[5]), cmul(u.m[2], u.m[4])));
+        u.m[7] = conjg(csub(cmul(u.m[2], u.m[3]), cmul(u.m[0], u.m[5])));
+        u.m[8] = conjg(csub(cmul(u.m[0], u.m[4]), cmul(u.m[1], u.m[3])));
+    }
+}
+template <typename C, typename M>
+__device__ __forceinline__ void store_u(C* __restrict__ U, size_t V, size_t i, const M& r) {
+#pragma unroll
+    for (int k = 0; k < UPlanes<C>::value; k++) U[k * V + i] = r.m[k];
+}
+
 // ordering fences: values passed through them are final at the fence and
 // values read after it are defined there (empty asm statements keep their order)
 __device__ __forceinline__ void fence(double& v) { asm volatile("" : "+v"(v)); }
@@ -301,7 +324,7 @@ __device__ __forceinline__ void row_uxiu_fwd_body(C* lds, BlockIx bi, C* __restr
     const int tid = bi.tid;
     const auto twl = tw_source<N, true>(tw);
     Vv += zofs_z(bi.bz, 8, V);
-    U += zofs_z(bi.bz, 9, V);
+    U += zofs_z(bi.bz, UPlanes<C>::value, V);
     ns = batch_noise_z(ns, V, bi.bz);
     const int x = row_of_block<N, BF>(bi.bx);
     const size_t row0 = (size_t)x * N;
@@ -318,8 +341,7 @@ __device__ __forceinline__ void row_uxiu_fwd_body(C* lds, BlockIx bi, C* __restr
         for (int k = 0; k < 8; k++) { hx[k] = z[2 * k] * s; hy[k] = z[2 * k + 1] * s; }
         // FP64 form: FP64 products; mixed form: FP32 products
         typename site_mat<C>::type u;
-#pragma unroll
-        for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+        load_u(u, U, V, i);
         T vx[8], v2[8];
         uxiu_gen(u, hx, vx);
         uxiu_gen(u, hy, v2);
@@ -377,7 +399,7 @@ __device__ __forceinline__ void row_uxiu_corr_body(C* lds, BlockIx bi, C* __rest
     const int tid = bi.tid;
     const auto twl = tw_source<N, false>(tw);
     Vv += zofs_z(bi.bz, 8, V);
-    U += zofs_z(bi.bz, 9, V);
+    U += zofs_z(bi.bz, UPlanes<C>::value, V);
     Xs += zofs_z(bi.bz, 8, V);
     const int x = row_of_block<N, BF>(bi.bx);
     const size_t row0 = (size_t)x * N;
@@ -403,8 +425,7 @@ __device__ __forceinline__ void row_uxiu_corr_body(C* lds, BlockIx bi, C* __rest
                 h[2 * p + 1] = a.y;
             }
             typename site_mat<C>::type u;
-#pragma unroll
-            for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+            load_u(u, U, V, i);
             T v[8];
             uxiu_gen(u, h, v);
 #pragma unroll
@@ -755,7 +776,7 @@ __device__ __forceinline__ void row_update_body(C* lds, BlockIx bi, C* __restric
     constexpr int PN = padded<N, 4>::value;     // padded row length in LDS
     const int tid = bi.tid;
     const auto twl = tw_source<N, sizeof(C) == sizeof(float2)>(tw);
-    U += zofs_z(bi.bz, 9, V);
+    U += zofs_z(bi.bz, UPlanes<C>::value, V);
     ZA += zofs_z(bi.bz, 4, V);
     ZB += zofs_z(bi.bz, 4, V);
     const int x = row_of_block<N, BF>(bi.bx);
@@ -815,12 +836,11 @@ __device__ __forceinline__ void row_update_body(C* lds, BlockIx bi, C* __restric
             // fences keep the two exponentials apart (fewer live registers).
             const m3f eb = exp_i_gen_f(hb, ss);
             m3f u;
-#pragma unroll
-            for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+            load_u(u, U, V, i);
             m3f w = mmul(eb, u);
             T hd[8];
 #pragma unroll
-            for (int k = 0; k < 9; k++) fence(w.m[k]);
+            for (int k = 0; k < UPlanes<C>::value; k++) fence(w.m[k]);      // the rows that are stored
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 hd[k] = ha[q][k];
@@ -828,8 +848,7 @@ __device__ __forceinline__ void row_update_body(C* lds, BlockIx bi, C* __restric
             }
             m3f r = mmul(w, exp_i_gen_f(hd, -ss));
             if (reunit) r = to_f(su3_project(to_d(r)));
-#pragma unroll
-            for (int k = 0; k < 9; k++) U[k * V + i] = r.m[k];
+            store_u(U, V, i, r);
         }
     }
     if (next_noise) {        // the next step's xi row spectra

@@ -162,7 +162,9 @@ hipError_t step_multi_f32(hipStream_t st, const GroupArgs* g, int ng, const Step
 bool large_supported(int N);
 hipError_t step_large(jw_ctx* c, cplx* U, cplx* Xi, cplx* Vv, cplx* ZA, cplx* ZB, const StepArgs& a);
 hipError_t step_large_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, cplxf* ZB, const StepArgs& a);
-hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32);
+// U (9 FP64 planes) to / from the mixed form's FP32 state of `planes` planes
+// (9, or 6: rows 0 and 1, row 2 rebuilt on the way back)
+hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32, int planes);
 hipError_t step_inc(hipStream_t s, uint32_t* step);
 hipError_t block16(jw_ctx* c, double* out, const double* in);
 

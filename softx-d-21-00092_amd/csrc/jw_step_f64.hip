@@ -37,7 +37,9 @@ hipError_t block16(jw_ctx* c, double* out, const double* in) {
 }
 
 // ---------------------------------------------------------------------------
-// U between the FP64 field and the FP32 state of the mixed-precision form
+// U between the FP64 field (9 planes) and the FP32 state of the mixed-precision
+// form: 9 planes, or rows 0 and 1 (6 planes, the fused passes) with row 2
+// rebuilt as conj(row 0 x row 1) in FP64 on the way back
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_u_to_f32(cplxf* __restrict__ out, const cplx* __restrict__ in, size_t n) {
     GRID_STRIDE(i, n) out[i] = mkf((float)in[i].x, (float)in[i].y);
@@ -45,11 +47,26 @@ __global__ __launch_bounds__(256) void k_u_to_f32(cplxf* __restrict__ out, const
 __global__ __launch_bounds__(256) void k_u_to_f64(cplx* __restrict__ out, const cplxf* __restrict__ in, size_t n) {
     GRID_STRIDE(i, n) out[i] = to_d(in[i]);
 }
+__global__ __launch_bounds__(256) void k_u2_to_f64(cplx* __restrict__ out, const cplxf* __restrict__ in, size_t V) {
+    GRID_STRIDE(i, V) {
+        cplx m[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            m[k] = to_d(in[k * V + i]);
+            out[k * V + i] = m[k];
+        }
+        out[6 * V + i] = conjg(csub(cmul(m[1], m[5]), cmul(m[2], m[4])));
+        out[7 * V + i] = conjg(csub(cmul(m[2], m[3]), cmul(m[0], m[5])));
+        out[8 * V + i] = conjg(csub(cmul(m[0], m[4]), cmul(m[1], m[3])));
+    }
+}
 
-hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32) {
-    const size_t n = 9 * c->V;
+hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32, int planes) {
+    if (planes != 6 && planes != 9) return hipErrorInvalidValue;
+    const size_t n = (size_t)planes * c->V;
     if (to_f32) hipLaunchKernelGGL(k_u_to_f32, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, u32, u64, n);
-    else hipLaunchKernelGGL(k_u_to_f64, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, u64, u32, n);
+    else if (planes == 9) hipLaunchKernelGGL(k_u_to_f64, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, u64, u32, n);
+    else hipLaunchKernelGGL(k_u2_to_f64, dim3(grid_for(c->V, 256)), dim3(256), 0, c->stream, u64, u32, c->V);
     return hipGetLastError();
 }
 
