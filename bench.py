@@ -51,26 +51,32 @@ METRIC = "Langevin steps/s at N=2048 SU(3): mixed+batched vs FP64; S(r) err vs F
 HBM_PEAK_GBS = 8000.0
 ALPHA_MAX = 0.7
 LAMBDA_A = 0.02
-# compulsory HBM reals per site per step (x 8 B in FP64, x 4 B in FP32):
-# 182 for per-mode correlated noise, 132 for the white-noise couplings
-REALS_PER_SITE = {"correlated": 182, "white": 132, "kspace": 150}
 # Algorithmic HBM traffic of each kernel class of the fused step, in complex
-# planes per launch (read + written; U is 9 planes):
+# planes per launch (read + written); U is u = 9 planes in FP64 and 6 in the
+# mixed form (rows 0 and 1, row 2 rebuilt on load):
 #   row_noise_fwd  W 8                                        8
 #   col_kmult      A: R 8 W 4 (+W 8 correlated), B: R 8 W 4   12 and 12 (20 and 12)
-#   row_uxiu_fwd   R U 9 (+ xi~ 8 correlated), W 8            17 (25)
-#   row_update     R A 4 + B 4 + U 9, W U 9                   26
+#   row_uxiu_fwd   R U u (+ xi~ 8 correlated), W 8            u + 8 (u + 16)
+#   row_update     R A 4 + B 4 + U u, W U u                   8 + 2u
 #                  (FP64: + W 8 when it also writes the next step's xi)
-# Per step: correlated 91 planes = 182 reals per site, the compulsory figure;
-# white noise 75 planes = 150 reals (U is read by both row_uxiu_fwd and
-# row_update).
+# Per step: correlated 64 + 3u planes (FP64 91 = 182 reals per site);
+# white noise 48 + 3u (U is read by both row passes).
 # Momentum-space noise (forms *_k): no noise pass; column pass A draws the
 # spectra and writes A (4) and the noise's row spectra (8), the row pass reads
-# them back as in the correlated step: 12 + 12 + 25 + 26 = 75 planes = 150 reals
-# per site for every coupling.
-PLANES = {"correlated": {"row_noise_fwd": 8, "col_kmult": (20, 12), "row_uxiu_fwd": 25, "row_update": 26},
-          "white": {"row_noise_fwd": 8, "col_kmult": (12, 12), "row_uxiu_fwd": 17, "row_update": 26},
-          "kspace": {"row_noise_fwd": 0, "col_kmult": (12, 12), "row_uxiu_fwd": 25, "row_update": 26}}
+# them back as in the correlated step: 48 + 3u planes for every coupling
+# (FP64 150 reals per site, mixed 132).
+def planes_of(kind, u):
+    t = {"correlated": {"row_noise_fwd": 8, "col_kmult": (20, 12), "row_uxiu_fwd": u + 16, "row_update": 8 + 2 * u},
+         "white": {"row_noise_fwd": 8, "col_kmult": (12, 12), "row_uxiu_fwd": u + 8, "row_update": 8 + 2 * u},
+         "kspace": {"row_noise_fwd": 0, "col_kmult": (12, 12), "row_uxiu_fwd": u + 16, "row_update": 8 + 2 * u}}[kind]
+    return t
+
+
+def reals_of(kind, u):
+    t = planes_of(kind, u)
+    return 2 * (t["row_noise_fwd"] + sum(t["col_kmult"]) + t["row_uxiu_fwd"] + t["row_update"])
+
+
 # measured HBM traffic per launch of the dominant kernels (rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a plain copy)
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_N2048.json")
@@ -212,16 +218,17 @@ def main():
         ctx.timing(False)
         bpr = 4 if form.startswith("mixed") else 8
         kind_f = "kspace" if form.endswith("_k") else kind
-        classes, roof = kernel_table(kt, nk, V, PLANES[kind_f], bpr)
+        u_pl = 6 if bpr == 4 else 9
+        classes, roof = kernel_table(kt, nk, V, planes_of(kind_f, u_pl), bpr)
         print(f"bench: {form} kernel classes {json.dumps({k: v['ms_per_step'] for k, v in classes.items()})}",
               file=sys.stderr, flush=True)
-        step_bytes = REALS_PER_SITE[kind_f] * bpr * V
+        step_bytes = reals_of(kind_f, u_pl) * bpr * V
         res[form] = {"steps_per_s": round(sps, 3), "ms_per_step": round(1e3 / sps, 4), "kernels": classes,
                      "roofline": roof,
                      "step_roofline": {"bytes_per_step": step_bytes,
                                        "achieved_GBs": round(step_bytes * sps / 1e9, 1),
                                        "frac": round(step_bytes * sps / (HBM_PEAK_GBS * 1e9), 4),
-                                       "reals_per_site": REALS_PER_SITE[kind_f], "bytes_per_real": bpr}}
+                                       "reals_per_site": reals_of(kind_f, u_pl), "bytes_per_real": bpr}}
         F.close()
 
     def time_batch(groups, fp32, coupling_b, step_b, al_b, nsteps, warm, seeds0, kspace=False, launch=-1):
@@ -319,7 +326,7 @@ def main():
             QL[name] = jw.saturation_scale(r_, s_, c_)
             F.close()
             bpr = 4 if name == "mixed" else 8
-            comp = REALS_PER_SITE["kspace" if ks else kind] * bpr * NL * NL
+            comp = reals_of("kspace" if ks else kind, 6 if name == "mixed" else 9) * bpr * NL * NL
             large[name] = {"steps_per_s": round(a.large_steps / dt, 3),
                            "alg_GBs": round(comp * a.large_steps / dt / 1e9, 1),
                            "alg_frac": round(comp * a.large_steps / dt / (HBM_PEAK_GBS * 1e9), 4), "Qs": QL[name]}
@@ -340,10 +347,11 @@ def main():
         F.close()
         large["sr_err_vs_fp64"] = float(np.abs(SL["fp64"] - SL["mixed"]).max())
         large["Qs_rel_err_vs_fp64"] = abs(QL["mixed"] - QL["fp64"]) / QL["fp64"] if QL["fp64"] else None
-        large["bytes_per_step_note"] = ("alg_GBs: the compulsory bytes of the fused step (%d reals per site); the "
-                                        "large-lattice step moves %d reals per site (row transforms and site "
-                                        "passes apart)" % (REALS_PER_SITE["kspace" if ks else kind],
-                                                           246 if ks else 294))
+        large["bytes_per_step_note"] = ("alg_GBs: the compulsory bytes of the fused step (FP64 %d, mixed %d reals "
+                                        "per site); the large-lattice step moves %d reals per site in FP64 (row "
+                                        "transforms and site passes apart)" % (
+                                            reals_of("kspace" if ks else kind, 9), reals_of("kspace" if ks else kind, 6),
+                                            246 if ks else 294))
         print(f"bench: N={NL} S(r) mixed vs fp64 {large['sr_err_vs_fp64']:.3e}, Q_s rel err "
               f"{large['Qs_rel_err_vs_fp64']}", file=sys.stderr, flush=True)
         cl.close()
