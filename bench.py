@@ -52,19 +52,19 @@ HBM_PEAK_GBS = 8000.0
 ALPHA_MAX = 0.7
 LAMBDA_A = 0.02
 # Algorithmic HBM traffic of each kernel class of the fused step, in complex
-# planes per launch (read + written); U is u = 9 planes in FP64 and 6 in the
-# mixed form (rows 0 and 1, row 2 rebuilt on load):
+# planes per launch (read + written); the fused passes read and write u = 6
+# planes of U (rows 0 and 1, row 2 rebuilt on load):
 #   row_noise_fwd  W 8                                        8
 #   col_kmult      A: R 8 W 4 (+W 8 correlated), B: R 8 W 4   12 and 12 (20 and 12)
 #   row_uxiu_fwd   R U u (+ xi~ 8 correlated), W 8            u + 8 (u + 16)
 #   row_update     R A 4 + B 4 + U u, W U u                   8 + 2u
 #                  (FP64: + W 8 when it also writes the next step's xi)
-# Per step: correlated 64 + 3u planes (FP64 91 = 182 reals per site);
-# white noise 48 + 3u (U is read by both row passes).
+# Per step: correlated 64 + 3u planes (82 = 164 reals per site); white noise
+# 48 + 3u (U is read by both row passes).
 # Momentum-space noise (forms *_k): no noise pass; column pass A draws the
 # spectra and writes A (4) and the noise's row spectra (8), the row pass reads
-# them back as in the correlated step: 48 + 3u planes for every coupling
-# (FP64 150 reals per site, mixed 132).
+# them back as in the correlated step: 48 + 3u planes = 132 reals per site for
+# every coupling.
 def planes_of(kind, u):
     t = {"correlated": {"row_noise_fwd": 8, "col_kmult": (20, 12), "row_uxiu_fwd": u + 16, "row_update": 8 + 2 * u},
          "white": {"row_noise_fwd": 8, "col_kmult": (12, 12), "row_uxiu_fwd": u + 8, "row_update": 8 + 2 * u},
@@ -218,7 +218,7 @@ def main():
         ctx.timing(False)
         bpr = 4 if form.startswith("mixed") else 8
         kind_f = "kspace" if form.endswith("_k") else kind
-        u_pl = 6 if bpr == 4 else 9
+        u_pl = 6
         classes, roof = kernel_table(kt, nk, V, planes_of(kind_f, u_pl), bpr)
         print(f"bench: {form} kernel classes {json.dumps({k: v['ms_per_step'] for k, v in classes.items()})}",
               file=sys.stderr, flush=True)
@@ -326,7 +326,7 @@ def main():
             QL[name] = jw.saturation_scale(r_, s_, c_)
             F.close()
             bpr = 4 if name == "mixed" else 8
-            comp = reals_of("kspace" if ks else kind, 6 if name == "mixed" else 9) * bpr * NL * NL
+            comp = reals_of("kspace" if ks else kind, 6) * bpr * NL * NL
             large[name] = {"steps_per_s": round(a.large_steps / dt, 3),
                            "alg_GBs": round(comp * a.large_steps / dt / 1e9, 1),
                            "alg_frac": round(comp * a.large_steps / dt / (HBM_PEAK_GBS * 1e9), 4), "Qs": QL[name]}
@@ -347,11 +347,10 @@ def main():
         F.close()
         large["sr_err_vs_fp64"] = float(np.abs(SL["fp64"] - SL["mixed"]).max())
         large["Qs_rel_err_vs_fp64"] = abs(QL["mixed"] - QL["fp64"]) / QL["fp64"] if QL["fp64"] else None
-        large["bytes_per_step_note"] = ("alg_GBs: the compulsory bytes of the fused step (FP64 %d, mixed %d reals "
-                                        "per site); the large-lattice step moves %d reals per site in FP64 (row "
-                                        "transforms and site passes apart)" % (
-                                            reals_of("kspace" if ks else kind, 9), reals_of("kspace" if ks else kind, 6),
-                                            246 if ks else 294))
+        large["bytes_per_step_note"] = ("alg_GBs: the compulsory bytes of the fused step (%d reals per site); the "
+                                        "large-lattice step moves %d reals per site (row transforms and site "
+                                        "passes apart, U in nine planes)" % (reals_of("kspace" if ks else kind, 6),
+                                                                             246 if ks else 294))
         print(f"bench: N={NL} S(r) mixed vs fp64 {large['sr_err_vs_fp64']:.3e}, Q_s rel err "
               f"{large['Qs_rel_err_vs_fp64']}", file=sys.stderr, flush=True)
         cl.close()

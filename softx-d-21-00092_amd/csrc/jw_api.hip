@@ -148,6 +148,8 @@ int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, bool 
         }
         HIP_CHECK(jw::step_unfused(c, U->d, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc, corr, ss));
     }
+    // the fused passes keep rows 0 and 1 of U current
+    if (fused && nsteps > 0) HIP_CHECK(jw::u_rebuild(c, U->d, c->stream));
     return JW_OK;
 }
 
@@ -294,8 +296,11 @@ struct jw_batch {
 namespace {
 
 size_t elem_size(const jw_batch* b) { return b->fp32 ? sizeof(jw::cplxf) : sizeof(cplx); }
-// U planes per trajectory: the mixed form's fused passes hold rows 0 and 1
-int u_planes(const jw_batch* b) { return b->fp32 ? 6 : 9; }
+// U planes per trajectory: the fused passes hold rows 0 and 1
+int u_planes(const jw_batch* b) {
+    (void)b;
+    return 6;
+}
 
 void batch_free(jw_batch* b) {
     if (!b) return;
@@ -1335,7 +1340,7 @@ int jw_batch_set(jw_batch* b, int t, const jw_field* U) {
     if (b->fp32) {
         HIP_CHECK(jw::u_convert(g->ctx, (jw::cplxf*)g->U + (size_t)k * 6 * V, U->d, 1, 6));
     } else {
-        HIP_CHECK(hipMemcpyAsync((cplx*)g->U + (size_t)k * 9 * V, U->d, 9 * V * sizeof(cplx),
+        HIP_CHECK(hipMemcpyAsync((cplx*)g->U + (size_t)k * 6 * V, U->d, 6 * V * sizeof(cplx),
                                  hipMemcpyDeviceToDevice, g->ctx->stream));
     }
     HIP_CHECK(hipStreamSynchronize(g->ctx->stream));
@@ -1354,8 +1359,9 @@ int jw_batch_get(jw_batch* b, int t, jw_field* U) {
     if (b->fp32) {
         HIP_CHECK(jw::u_convert(g->ctx, (jw::cplxf*)g->U + (size_t)k * 6 * V, U->d, 0, 6));
     } else {
-        HIP_CHECK(hipMemcpyAsync(U->d, (cplx*)g->U + (size_t)k * 9 * V, 9 * V * sizeof(cplx),
+        HIP_CHECK(hipMemcpyAsync(U->d, (cplx*)g->U + (size_t)k * 6 * V, 6 * V * sizeof(cplx),
                                  hipMemcpyDeviceToDevice, g->ctx->stream));
+        HIP_CHECK(jw::u_rebuild(g->ctx, U->d, g->ctx->stream));
     }
     HIP_CHECK(hipStreamSynchronize(g->ctx->stream));
     return JW_OK;

@@ -136,13 +136,14 @@ struct site_mat { typedef m3 type; };
 template <>
 struct site_mat<float2> { typedef m3f type; };
 
-// U planes held per trajectory by the fused passes: FP64 all nine; the mixed
-// form rows 0 and 1 (six planes), row 2 rebuilt on load as conj(row 0 x row 1)
-// -- U is in SU(3) to FP32 rounding and projected every few steps, and the
-// projection rebuilds row 2 the same way -- so that the row passes move a third
-// less U.
+// U planes read and written by the fused passes: rows 0 and 1 (six planes), row
+// 2 rebuilt on load as conj(row 0 x row 1) -- U is in SU(3) to the rounding of
+// its form (and the mixed form projects it every few steps, rebuilding row 2
+// the same way) -- so that the row passes move a third less U.  The FP64 field
+// keeps its nine planes; row 2 is rebuilt into it after the fused steps
+// (u_rebuild).
 template <typename C>
-struct UPlanes { static constexpr int value = sizeof(C) == sizeof(float2) ? 6 : 9; };
+struct UPlanes { static constexpr int value = 6; };
 template <typename C, typename M>
 __device__ __forceinline__ void load_u(M& u, const C* __restrict__ U, size_t V, size_t i) {
 #pragma unroll
@@ -816,12 +817,11 @@ __device__ __forceinline__ void row_update_body(C* lds, BlockIx bi, C* __restric
         }
         if constexpr (sizeof(C) == sizeof(double2)) {
             m3 u;
-#pragma unroll
-            for (int k = 0; k < 9; k++) u.m[k] = U[k * V + i];
+            load_u(u, U, V, i);
             m3 w = mmul(exp_i_gen(hb, ss), u);
             double hd[8];
 #pragma unroll
-            for (int k = 0; k < 9; k++) fence(w.m[k]);
+            for (int k = 0; k < UPlanes<C>::value; k++) fence(w.m[k]);      // the rows that are stored
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 hd[k] = ha[q][k];
@@ -829,8 +829,7 @@ __device__ __forceinline__ void row_update_body(C* lds, BlockIx bi, C* __restric
             }
             m3 r = mmul(w, exp_i_gen(hd, -ss));
             if (reunit) r = su3_project(r);
-#pragma unroll
-            for (int k = 0; k < 9; k++) U[k * V + i] = r.m[k];
+            store_u(U, V, i, r);
         } else {
             // FP64 exponentials, FP32 products; the projection in FP64.  The
             // fences keep the two exponentials apart (fewer live registers).

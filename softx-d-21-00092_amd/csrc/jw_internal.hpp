@@ -165,6 +165,8 @@ hipError_t step_large_f32(jw_ctx* c, cplxf* U, cplxf* Xi, cplxf* Vv, cplxf* ZA, 
 // U (9 FP64 planes) to / from the mixed form's FP32 state of `planes` planes
 // (9, or 6: rows 0 and 1, row 2 rebuilt on the way back)
 hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32, int planes);
+// row 2 of a 9-plane FP64 U rebuilt from rows 0 and 1 (on stream s)
+hipError_t u_rebuild(jw_ctx* c, cplx* U, hipStream_t s);
 hipError_t step_inc(hipStream_t s, uint32_t* step);
 hipError_t block16(jw_ctx* c, double* out, const double* in);
 

@@ -61,6 +61,24 @@ __global__ __launch_bounds__(256) void k_u2_to_f64(cplx* __restrict__ out, const
     }
 }
 
+// row 2 of an FP64 field rebuilt from rows 0 and 1 (after the fused passes,
+// which keep only those rows current)
+__global__ __launch_bounds__(256) void k_u_rebuild(cplx* __restrict__ U, size_t V) {
+    GRID_STRIDE(i, V) {
+        cplx m[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) m[k] = U[k * V + i];
+        U[6 * V + i] = conjg(csub(cmul(m[1], m[5]), cmul(m[2], m[4])));
+        U[7 * V + i] = conjg(csub(cmul(m[2], m[3]), cmul(m[0], m[5])));
+        U[8 * V + i] = conjg(csub(cmul(m[0], m[4]), cmul(m[1], m[3])));
+    }
+}
+
+hipError_t u_rebuild(jw_ctx* c, cplx* U, hipStream_t s) {
+    hipLaunchKernelGGL(k_u_rebuild, dim3(grid_for(c->V, 256)), dim3(256), 0, s, U, c->V);
+    return hipGetLastError();
+}
+
 hipError_t u_convert(jw_ctx* c, cplxf* u32, cplx* u64, int to_f32, int planes) {
     if (planes != 6 && planes != 9) return hipErrorInvalidValue;
     const size_t n = (size_t)planes * c->V;
