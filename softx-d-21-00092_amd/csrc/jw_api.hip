@@ -148,8 +148,8 @@ int evolve_generator_basis(jw_ctx* c, jw_field* U, int nsteps, bool fused, bool 
         }
         HIP_CHECK(jw::step_unfused(c, U->d, Xi, Vv, ZA, ZB, ns, sa_pos, sa_k, kc, corr, ss));
     }
-    // the fused passes keep rows 0 and 1 of U current
-    if (fused && nsteps > 0) HIP_CHECK(jw::u_rebuild(c, U->d, c->stream));
+    // the fused and large-lattice passes keep rows 0 and 1 of U current
+    if ((fused || large) && nsteps > 0) HIP_CHECK(jw::u_rebuild(c, U->d, c->stream));
     return JW_OK;
 }
 
@@ -1076,8 +1076,9 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
         rc = ensure_sa16(c);
         sa_k = c->sa16;
     }
-    // the fused passes hold rows 0 and 1 of U (row 2 rebuilt), the others all nine entries
-    const int uplanes = fused ? 6 : 9;
+    // the fused and large-lattice passes hold rows 0 and 1 of U (row 2 rebuilt),
+    // the unfused sequence all nine entries
+    const int uplanes = fused || large ? 6 : 9;
     if (rc == JW_OK && jw::u_convert(c, U32, U->d, 1, uplanes) != hipSuccess) rc = fail(JW_ERR_HIP, "U conversion failed");
     for (int it = 0; rc == JW_OK && it < nsteps; it++) {
         bool ok;

@@ -184,7 +184,20 @@ __global__ __launch_bounds__(256) void k_kernel_mult_packed(C* __restrict__ Z, C
     }
 }
 
-template <typename C>
+// U for the FP64 site algebra of the packed kernels: all nine planes, or (U6,
+// the large-lattice step) rows 0 and 1 with row 2 rebuilt as conj(row 0 x row 1)
+template <int U6, typename C>
+__device__ __forceinline__ void load_u_d(m3& u, const C* __restrict__ U, size_t V, size_t i) {
+#pragma unroll
+    for (int k = 0; k < (U6 ? 6 : 9); k++) u.m[k] = to_d(U[k * V + i]);
+    if constexpr (U6 != 0) {
+        u.m[6] = conjg(csub(cmul(u.m[1], u.m[5]), cmul(u.m[2], u.m[4])));
+        u.m[7] = conjg(csub(cmul(u.m[2], u.m[3]), cmul(u.m[0], u.m[5])));
+        u.m[8] = conjg(csub(cmul(u.m[0], u.m[4]), cmul(u.m[1], u.m[3])));
+    }
+}
+
+template <typename C, int U6 = 0>
 __global__ __launch_bounds__(256) void k_uxiu_packed(C* __restrict__ Vv, const C* __restrict__ U,
                                                      const C* __restrict__ Xi_pos, size_t V, NoiseSrc ns,
                                                      const double* __restrict__ sa_pos) {
@@ -206,8 +219,7 @@ __global__ __launch_bounds__(256) void k_uxiu_packed(C* __restrict__ Vv, const C
             for (int k = 0; k < 8; k++) { hx[k] = z[2 * k] * s; hy[k] = z[2 * k + 1] * s; }
         }
         m3 u;
-#pragma unroll
-        for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
+        load_u_d<U6>(u, U, V, i);
         double vx[8], vy[8];
         uxiu_gen(u, hx, vx);
         uxiu_gen(u, hy, vy);
@@ -219,7 +231,7 @@ __global__ __launch_bounds__(256) void k_uxiu_packed(C* __restrict__ Vv, const C
     }
 }
 
-template <typename C>
+template <typename C, int U6 = 0>
 __global__ __launch_bounds__(256) void k_update_packed(C* __restrict__ U, const C* __restrict__ ZA,
                                                        const C* __restrict__ ZB, size_t V, double ss, int reunit) {
     GRID_STRIDE(i, V) {
@@ -233,12 +245,11 @@ __global__ __launch_bounds__(256) void k_update_packed(C* __restrict__ U, const 
         const m3 eb = exp_i_gen(hb, ss);
         const m3 ea = exp_i_gen(ha, -ss);
         m3 u;
-#pragma unroll
-        for (int k = 0; k < 9; k++) u.m[k] = to_d(U[k * V + i]);
+        load_u_d<U6>(u, U, V, i);
         m3 r = mmul(mmul(eb, u), ea);
         if (reunit) r = su3_project(r);
 #pragma unroll
-        for (int k = 0; k < 9; k++) U[k * V + i] = mkc<C>(r.m[k].x, r.m[k].y);
+        for (int k = 0; k < (U6 ? 6 : 9); k++) U[k * V + i] = mkc<C>(r.m[k].x, r.m[k].y);
     }
 }
 
@@ -295,19 +306,97 @@ static hipError_t step_unfused_t(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, co
 // (forward column FFT, kernel multiply, inverse column FFT in LDS per strip;
 // the momentum-space noise drawn there) with the row transforms and the site
 // algebra as separate passes, since one lattice row of four planes no longer
-// fits LDS (FP64 at N = 4096 needs 256 KB).  Per step, in complex planes moved:
+// fits LDS (FP64 at N = 4096 needs 256 KB).  U is held as rows 0 and 1 (row 2
+// rebuilt on load), as in the fused step.  Per step, in complex planes moved:
 //   [position-space noise: noise W 8, forward row FFT R 8 W 8]
 //   column A                     R 8 (0 drawn) W 4 (+8 correlated or drawn)
-//   inverse row FFT of A (and xi~) R 4 (+8) W 4 (+8)
-//   U xi U^dagger                R 9 (+8) W 8
-//   forward row FFT of V         R 8 W 8
+//   inverse row FFT of A         R 4 W 4
+//   correlated or drawn noise, k_row_uxiu_1p (FP32 at N = 8192):
+//     inverse row FFT of xi~, U xi~ U^dagger, forward row FFT of V  R 20 W 8
+//   otherwise: [inverse row FFT of xi~ R 8 W 8]
+//     U xi U^dagger              R 6 (+8) W 8
+//     forward row FFT of V       R 8 W 8
 //   column B                     R 8 W 4
 //   inverse row FFT of B         R 4 W 4
-//   update                       R 17 W 9
-// 123 planes with the noise drawn per mode, against 75 for the fused step of
-// N <= 2048.  Column strips: BF = 32 (16-byte row segments) at N = 8192,
-// BF = 0 or 32 at N = 4096 (the intermediate planes stay row-major).
+//   update                       R 14 W 6
+// 88 planes with the noise drawn per mode in FP32 at N = 8192 (114 otherwise),
+// against 66 for the fused step of N <= 2048.  Column strips: BF = 32 (16-byte
+// row segments) at N = 8192, BF = 0 or 32 at N = 4096 (the intermediate planes
+// stay row-major).
 // ===========================================================================
+// U xi~ U^dagger of the large-lattice step as one pass over a lattice row (FP32
+// at N = 8192; FP32 at 4096 and FP64 spill registers), where four planes of a row do not fit
+// LDS: each plane of xi~ is inverse-transformed alone in LDS and its values at
+// the thread's SPT sites are kept in registers; with the four planes of a
+// direction held, V = U xi~ U^dagger per site (U as rows 0 and 1, read once per
+// direction), and the four planes of V go through LDS one at a time for the
+// forward row transform.  In place of the inverse row FFT of xi~ (R 8 W 8), the
+// site pass (R 14 W 8) and the forward row FFT of V (R 8 W 8): R 20 W 8.
+template <int N, typename C>
+struct Row1P {
+    static constexpr int NT = 1024, SPT = N / NT;
+    static constexpr bool ok = N == 8192 && sizeof(C) == sizeof(float2);
+};
+template <int N, typename C>
+__global__ __launch_bounds__(1024, 1) void k_row_uxiu_1p(C* __restrict__ Vv, const C* __restrict__ U,
+                                                          const C* __restrict__ Xs, size_t V,
+                                                          const C* __restrict__ tw) {
+    typedef typename scalar_of<C>::type T;
+    constexpr int NT = Row1P<N, C>::NT, SPT = Row1P<N, C>::SPT;
+    constexpr int PN = padded<N, 4>::value;
+    static_assert(SPT * NT == N, "row tiling");
+    __shared__ C lds[PN];
+    const int tid = threadIdx.x;
+    const size_t row0 = (size_t)blockIdx.x * N;
+#pragma unroll 1
+    for (int half = 0; half < 2; half++) {
+        C h[4][SPT];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const C* src = Xs + (size_t)(4 * half + p) * V + row0;
+            const int tl = opaque_v(tid);
+#pragma unroll
+            for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tl + q * NT, 0)] = src[tl + q * NT];
+            __syncthreads();
+            lds_fft<N, 1, 1, PN, NT, +1, 4>(lds, tw_fft(tw), opaque_v(tid));
+            const int tr = opaque_v(tid);
+#pragma unroll
+            for (int q = 0; q < SPT; q++) h[p][q] = lds[lpos<1, PN, 4>(tr + q * NT, 0)];
+            __syncthreads();
+        }
+        const int tu = opaque_v(tid);
+#pragma unroll
+        for (int q = 0; q < SPT; q++) {
+            const size_t i = row0 + tu + q * NT;
+            T hh[8];
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                hh[2 * p] = h[p][q].x;
+                hh[2 * p + 1] = h[p][q].y;
+            }
+            typename site_mat<C>::type u;
+            load_u(u, U, V, i);
+            T v[8];
+            uxiu_gen(u, hh, v);
+#pragma unroll
+            for (int p = 0; p < 4; p++) h[p][q] = mkc<C>(v[2 * p], v[2 * p + 1]);
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const int tl = opaque_v(tid);
+#pragma unroll
+            for (int q = 0; q < SPT; q++) lds[lpos<1, PN, 4>(tl + q * NT, 0)] = h[p][q];
+            __syncthreads();
+            lds_fft<N, 1, 1, PN, NT, -1, 4>(lds, tw_fft(tw), opaque_v(tid));
+            C* dst = Vv + (size_t)(4 * half + p) * V + row0;
+            const int ts = opaque_v(tid);
+#pragma unroll
+            for (int q = 0; q < SPT; q++) dst[ts + q * NT] = lds[lpos<1, PN, 4>(ts + q * NT, 0)];
+            __syncthreads();
+        }
+    }
+}
+
 template <int N, int BF, typename C>
 static hipError_t step_large_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, const StepArgs& a) {
     const size_t V = c->V;
@@ -329,14 +418,25 @@ static hipError_t step_large_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, cons
         JW_TRY((col_pass<N, BF>(c, ZA, Xi, Xi + 4 * V, a.sa_k, a.sqrt_coupling, a.corr, 1, a.ns, a.kspace, a.corr)));
     }
     JW_TRY((launch_rows<N, C>(c, ZA, 4, 0, 1.0, tw)));
-    if (wb) JW_TRY((launch_rows<N, C>(c, Xi, 8, 0, 1.0, tw)));
-    {
-        KTimer t(c, K_UXIU);
-        hipLaunchKernelGGL(k_uxiu_packed<C>, dim3(g), dim3(256), 0, c->stream, Vv, U, wb ? Xi : nullptr, V, a.ns,
-                           a.sa_pos);
-        JW_TRY(hipGetLastError());
+    bool one_pass = false;
+    if constexpr (Row1P<N, C>::ok) {
+        if (wb) {
+            KTimer t(c, K_UXIU);
+            hipLaunchKernelGGL((k_row_uxiu_1p<N, C>), dim3(N), dim3(Row1P<N, C>::NT), 0, c->stream, Vv, U, Xi, V, tw);
+            JW_TRY(hipGetLastError());
+            one_pass = true;
+        }
     }
-    JW_TRY((launch_rows<N, C>(c, Vv, 8, 1, 1.0, tw)));
+    if (!one_pass) {
+        if (wb) JW_TRY((launch_rows<N, C>(c, Xi, 8, 0, 1.0, tw)));
+        {
+            KTimer t(c, K_UXIU);
+            hipLaunchKernelGGL((k_uxiu_packed<C, 1>), dim3(g), dim3(256), 0, c->stream, Vv, U, wb ? Xi : nullptr, V,
+                               a.ns, a.sa_pos);
+            JW_TRY(hipGetLastError());
+        }
+        JW_TRY((launch_rows<N, C>(c, Vv, 8, 1, 1.0, tw)));
+    }
     {
         KTimer t(c, K_COL_KMULT);
         JW_TRY((col_pass<N, BF>(c, ZB, Vv, Vv + 4 * V, a.sa_k, a.sqrt_coupling, 0, 1, a.ns, 0, 0)));
@@ -344,7 +444,8 @@ static hipError_t step_large_n(jw_ctx* c, C* U, C* Xi, C* Vv, C* ZA, C* ZB, cons
     JW_TRY((launch_rows<N, C>(c, ZB, 4, 0, 1.0, tw)));
     {
         KTimer t(c, K_UPDATE);
-        hipLaunchKernelGGL(k_update_packed<C>, dim3(g), dim3(256), 0, c->stream, U, ZA, ZB, V, a.sqrt_step, a.reunit);
+        hipLaunchKernelGGL((k_update_packed<C, 1>), dim3(g), dim3(256), 0, c->stream, U, ZA, ZB, V, a.sqrt_step,
+                           a.reunit);
         JW_TRY(hipGetLastError());
     }
 #undef JW_TRY
