@@ -348,9 +348,10 @@ def main():
         large["sr_err_vs_fp64"] = float(np.abs(SL["fp64"] - SL["mixed"]).max())
         large["Qs_rel_err_vs_fp64"] = abs(QL["mixed"] - QL["fp64"]) / QL["fp64"] if QL["fp64"] else None
         large["bytes_per_step_note"] = ("alg_GBs: the compulsory bytes of the fused step (%d reals per site); the "
-                                        "large-lattice step moves %d reals per site (row transforms and site "
-                                        "passes apart, U in nine planes)" % (reals_of("kspace" if ks else kind, 6),
-                                                                             246 if ks else 294))
+                                        "large-lattice step moves %d reals per site in FP32 and %d in FP64 (row "
+                                        "transforms of A and B apart; FP64: also those of the noise and V)"
+                                        % (reals_of("kspace" if ks else kind, 6), 176 if ks else 240,
+                                           228 if ks else 292))
         print(f"bench: N={NL} S(r) mixed vs fp64 {large['sr_err_vs_fp64']:.3e}, Q_s rel err "
               f"{large['Qs_rel_err_vs_fp64']}", file=sys.stderr, flush=True)
         cl.close()
