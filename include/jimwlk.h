@@ -193,7 +193,7 @@ int jw_evolve(jw_ctx* ctx, jw_field* U, int coupling, int kernel, double step, c
  * main_optimized.cpp:234-311): U and every field and FFT in FP32, the SU(3)
  * exponentials and the projection of U onto SU(3) in FP64; the products
  * around the exponentials and U xi U^dagger are FP32 in the fused passes
- * (64 <= N <= 2048) and FP64 in the others.  U is
+ * (64 <= N <= 2048; U xi U^dagger also at N = 8192) and FP64 in the others.  U is
  * converted to FP32 on entry and back on return, and projected onto SU(3)
  * every reunit_every steps and after the last step (reunit_every = 0: never).
  * The noise is the same Philox stream (normals computed in FP32 from the same

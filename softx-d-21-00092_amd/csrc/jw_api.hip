@@ -1047,8 +1047,9 @@ int jw_evolve_mixed(jw_ctx* c, jw_field* U, int coupling, int kernel, double ste
     if (nsteps < 0 || reunit_every < 0) return fail(JW_ERR_ARG, "nsteps or reunit_every < 0");
     if (c->inj_per_site && nsteps > 1) return fail(JW_ERR_STATE, "an injection covers one step; nsteps = %d", nsteps);
     if (nsteps == 0) return JW_OK;
-    // FP32 state: U (9 planes) and the 24 intermediate planes, inside the
-    // workspace of 24 FP64 planes (48 FP32 planes)
+    // FP32 state: U (9 planes of room; 6 used by the fused and large-lattice
+    // passes) and the 24 intermediate planes, inside the workspace of 24 FP64
+    // planes (48 FP32 planes)
     if (int e = ensure_ws(c, 24)) return e;
     const size_t V = c->V;
     jw::cplxf* U32 = reinterpret_cast<jw::cplxf*>(c->ws);

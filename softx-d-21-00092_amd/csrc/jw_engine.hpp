@@ -2,16 +2,18 @@
 // templated on the storage type C (double2: FP64 form, float2: mixed form);
 // instantiated by jw_step_f64.hip and jw_step_f32.hip.
 //
-// One step (main_optimized.cpp:243-304 in the generator basis) as six passes
+// One step (main_optimized.cpp:243-304 in the generator basis) as five passes
 // over HBM instead of the reference's 8 nine-component FFTs and 5 site loops:
 //   row_noise_fwd  : Philox/injected noise -> xi planes, forward row FFT    W 8
 //   col_kmult (A)  : forward column FFT, kernel multiply, inverse column FFT R 8 W 4
 //                    (correlated noise: also writes F_col^-1[sqrt(alpha) F xi])
-//   row_uxiu       : V = U xi U^dagger, forward row FFT                      R 9 W 8
-//                    (correlated noise: inverse row FFT of xi~ first)     (R 17 W 8)
+//   row_uxiu       : V = U xi U^dagger, forward row FFT                      R 6 W 8
+//                    (correlated noise: inverse row FFT of xi~ first)     (R 14 W 8)
 //   col_kmult (B)  : as for A                                                R 8 W 4
-//   row_update     : inverse row FFTs of A, B; U <- exp(i s B) U exp(-i s A) R 17 W 9
-// (units: complex planes of V doubles pairs, 16 B per site each).
+//   row_update     : inverse row FFTs of A, B; U <- exp(i s B) U exp(-i s A) R 14 W 6
+// (units: complex planes of V elements, 16 B (FP64) or 8 B (FP32) per site each;
+// U as its rows 0 and 1, UPlanes below).  With the spectra drawn per mode the
+// noise pass goes and column A reads nothing and writes 12 planes.
 //
 // Row passes: one workgroup per lattice row, four planes at a time in LDS.
 //
@@ -413,7 +415,7 @@ __device__ __forceinline__ void row_uxiu_corr_body(C* lds, BlockIx bi, C* __rest
                 lds[lpos<1, PN, 4>(tid + q * NT, p)] = Xs[pat<N, BF>(4 * half + p, x, tid + q * NT, V)];
         __syncthreads();
         lds_fft<N, 4, 1, PN, NT, +1, 4>(lds, twl, tid);
-        // one site at a time: keeps a single U (9 complex) live
+        // one site at a time: keeps a single U (9 complex, row 2 rebuilt) live
 #pragma unroll 1
         for (int q = 0; q < SPT; q++) {
             const int y = tid + q * NT;

@@ -38,7 +38,7 @@ hipError_t block16(jw_ctx* c, double* out, const double* in) {
 
 // ---------------------------------------------------------------------------
 // U between the FP64 field (9 planes) and the FP32 state of the mixed-precision
-// form: 9 planes, or rows 0 and 1 (6 planes, the fused passes) with row 2
+// form: 9 planes, or rows 0 and 1 (6 planes, the fused and large-lattice passes) with row 2
 // rebuilt as conj(row 0 x row 1) in FP64 on the way back
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_u_to_f32(cplxf* __restrict__ out, const cplx* __restrict__ in, size_t n) {
