@@ -25,7 +25,9 @@ from the same MV start and Philox stream (max |S_mixed(r) - S_fp64(r)| over all
 lattice separations r) sit beside it, and every form's steps/s in
 `forms_steps_per_s`.  `ensemble` times
 the uneven batched ensemble 48 x 256^2 + 32 x 512^2 + 16 x 1024^2 (fixed
-coupling, Delta s = 1e-4) in both precisions, in site-updates/s.
+coupling, Delta s = 1e-4) in both precisions, in site-updates/s; `production`
+the FP64 batch of 16 seeds of N = 1024 (fixed coupling, Delta s = 1e-4);
+`large` the N = 8192 single lattice (correlated coupling) in both forms.
 
 One step = noise generation, 2-D FFTs, kernel multiplies, U xi U^dagger and
 the SU(3) update; measurement is outside the timed region, as the reference's
@@ -33,7 +35,7 @@ the SU(3) update; measurement is outside the timed region, as the reference's
 
 Also printed: `roofline` for the dominant kernel of the mixed form (HIP-event
 timing on the library's stream; `roofline_fp64` likewise), per-kernel tables,
-and `cpu_baseline` (the CPU oracle timed on one step of the same workload on
+and `cpu_baseline` (the CPU oracle timed on steps of the same workload on
 this host).
 """
 import argparse
@@ -109,6 +111,8 @@ def parse():
                    help="noise of the headline forms, the batched form and the S(r) comparison")
     p.add_argument("--batch", type=int, default=2, help="N = 2048 trajectories of the batched form (0: skip)")
     p.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble")
+    p.add_argument("--no-production", action="store_true",
+                   help="skip the FP64 production batch (16 trajectories of N = 1024, fixed coupling)")
     p.add_argument("--large-steps", type=int, default=200,
                    help="steps of the N = 8192 single-lattice configuration in both forms (0: skip)")
     return p.parse_args()
@@ -289,6 +293,25 @@ def main():
             print(f"bench: ensemble {name}: {sps_e:.1f} steps/s = {sps_e * sites:.3e} site-updates/s "
                   f"(layouts {lays}, launch mode {mode})", file=sys.stderr, flush=True)
         ensemble["mixed_over_fp64"] = round(ensemble["mixed"]["steps_per_s"] / ensemble["fp64"]["steps_per_s"], 3)
+
+    # the FP64 production run: 16 seeds of N = 1024 (MV start Ny = 50, g^2 mu a = 0.25,
+    # m a = 1e-4), fixed coupling, Delta s = 1e-4, advanced together as one batch
+    production = None
+    if not a.no_production:
+        groups = [(1024, 16)]
+        sites = sum(n * n * c for n, c in groups)
+        sps_p, lays, mode = time_batch(groups, False, jw.FIXED_COUPLING, 1e-4, None, 20, 10, 1, ks)
+        kind_p = "kspace" if ks else "white"
+        gbs = reals_of(kind_p, 6) * 8 * sites * sps_p / 1e9
+        production = {"groups": [{"N": n, "count": c} for n, c in groups], "sites": sites, "form": "fp64",
+                      "coupling": "fixed", "step": 1e-4, "noise_space": a.noise_space,
+                      "steps_per_s": round(sps_p, 3), "traj_steps_per_s": round(sps_p * 16, 2),
+                      "site_updates_per_s": round(sps_p * sites, 1), "alg_GBs": round(gbs, 1),
+                      "alg_frac": round(gbs / HBM_PEAK_GBS, 4), "reals_per_site": reals_of(kind_p, 6),
+                      "layouts": lays, "launch": mode}
+        print(f"bench: production fp64 16 x N=1024: {sps_p * 16:.1f} trajectory-steps/s = "
+              f"{sps_p * sites:.3e} site-updates/s, {gbs:.0f} GB/s algorithmic (layouts {lays}, launch {mode})",
+              file=sys.stderr, flush=True)
 
     # the large single lattice: N = 8192, MV start Ny = 100, g^2 mu a = 0.05,
     # correlated coupling, Delta s = 5e-5; both forms from the same start and
@@ -474,6 +497,7 @@ def main():
            "mixed_over_fp64": round(res[fmix]["steps_per_s"] / res[f64]["steps_per_s"], 3)
            if fmix in res and f64 in res else None,
            "ensemble": ensemble,
+           "production": production,
            "large": large,
            "mfma": mfma,
            "sr_err_vs_fp64": sr,
