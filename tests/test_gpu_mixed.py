@@ -99,11 +99,15 @@ def test_mixed_dipole_after_1000_steps(jw, N, coupling):
 
 
 @pytest.mark.parametrize("N", [64, 512, 2048])
-@pytest.mark.parametrize("coupling", [3, 4])
-def test_mixed_layouts_agree(jw, N, coupling):
+@pytest.mark.parametrize("coupling,space", [(3, 0), (4, 0), (3, 1), (4, 1)])
+def test_mixed_layouts_agree(jw, N, coupling, space):
+    """Every intermediate layout of the mixed form gives the row-major result to FP32
+    rounding, with position- and momentum-space noise (the pair-interleaved layout
+    runs its own row pass for the correlated and drawn noise)."""
     c = jw.Context(N)
     U0 = mv_start(jw, c, Ny=2)
     al = alpha_for(jw, N, coupling)
+    c.set_noise_space(space)
     out = {}
     for mode in (c.FUSED_ROWMAJOR, c.FUSED_BLOCK4, c.FUSED_BLOCK8, c.FUSED_ROWMAJOR_WIDE, c.FUSED_INTERLEAVED,
                  c.FUSED_ROWMAJOR_NARROW, c.FUSED):
